@@ -1,0 +1,220 @@
+/*
+ * hyperball.h -- C ABI of the MI355X (gfx950) HyperBall engine.
+ *
+ * Drop-in boundary for the HyperBall path of the reference crate
+ * vigna/webgraph-algo-rs (webgraph-algo 0.1.0).  Every entry point names the
+ * reference item it replaces (paths relative to the reference's root).  Plain
+ * pointers and sizes only; no C++ exception crosses this boundary.  Every
+ * function returns an hb_status; hb_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * Ownership: graphs and weights passed to hb_create are borrowed only for the
+ * duration of the call (they are copied to device memory).  Counters,
+ * frontier bitmaps and accumulators live in device memory owned by the
+ * instance and are released by hb_destroy.  Output buffers belong to the
+ * caller.
+ *
+ * Threading: an instance is not thread safe (the reference takes &mut self);
+ * distinct instances may run concurrently, each on its own HIP stream.
+ */
+#ifndef HYPERBALL_H
+#define HYPERBALL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (anyhow::Result / panics in the reference) ------------ */
+typedef int hb_status;
+#define HB_OK 0
+#define HB_E_NOT_RUN 1       /* ensure_iteration, hyperball_impl.rs:624-630 */
+#define HB_E_NOT_REQUESTED 2 /* bail! for outputs not requested, :645 :655 :669 :684 :710 :730 */
+#define HB_E_INVALID_ARG 3   /* builder ensure! failures and former panics */
+#define HB_E_OOM 4           /* device or host allocation failed */
+#define HB_E_HIP 5           /* HIP runtime error */
+#define HB_E_NO_DEVICE 6     /* no usable gfx950 device */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *hb_last_error(void);
+
+/* ---- graphs -------------------------------------------------------------- */
+
+/* A graph in CSR form: successors of node v are succ[offsets[v] ..
+ * offsets[v+1]), sorted increasingly.  Replaces the reference's
+ * RandomAccessGraph::successors / num_nodes / num_arcs (external crate
+ * webgraph 0.2.0, used at hyperball_impl.rs:1037, :1119, :1123) and the
+ * cumulative outdegree function `D: Succ` (hyperball_impl.rs:1012), which is
+ * `offsets` itself. */
+typedef struct {
+  uint64_t num_nodes;
+  uint64_t num_arcs;
+  const uint64_t *offsets;   /* num_nodes + 1 entries */
+  const uint32_t *successors; /* num_arcs entries */
+} hb_csr;
+
+/* A graph whose arrays were allocated by this library (free with
+ * hb_graph_free). */
+typedef struct {
+  uint64_t num_nodes;
+  uint64_t num_arcs;
+  uint64_t *offsets;
+  uint32_t *successors;
+} hb_graph;
+
+/* Decodes a BVGraph (version 0, big-endian bit streams) from
+ * <basename>.properties and <basename>.graph into CSR.  Replaces
+ * BvGraph::with_basename(..).load() + successor decoding (webgraph 0.2.0;
+ * tests/test_hyperball.rs:226-227).  Codes: gamma, delta, unary, zeta_k. */
+hb_status hb_bvgraph_load(const char *basename, hb_graph *out);
+
+/* Decodes <basename>.offsets (gamma-coded bit-offset gaps) into num_nodes+1
+ * bit offsets; used to check a decode against the offsets the writer
+ * recorded. */
+hb_status hb_bvgraph_load_offsets(const char *basename, uint64_t num_nodes, uint64_t *out);
+
+/* Bit offset at which node v's record starts, as observed by the decoder of
+ * hb_bvgraph_load; num_nodes+1 entries. */
+hb_status hb_bvgraph_decoder_offsets(const char *basename, uint64_t *out, uint64_t cap);
+
+/* Transpose by counting sort; rows of the result are sorted.  Replaces the
+ * `-t` transposed BvGraph the reference loads for systolic iterations
+ * (hyperball_impl.rs:204-256). */
+hb_status hb_graph_transpose(const hb_csr *g, hb_graph *out);
+
+/* Synthetic graphs (SplitMix64-seeded xoshiro256**, per-node streams, so the
+ * result does not depend on the host thread count).
+ *   uniform: every node gets exactly k distinct successors drawn uniformly
+ *            from [0,n) minus itself (Floyd sampling).
+ *   web:     out-degree Zipf(alpha) on [dmin, dmax]; each drawn successor is
+ *            uniform in the window [v-window, v+window] minus v (wrapping
+ *            mod n) with probability p_window, otherwise the target of a
+ *            uniformly random earlier arc (copy model, preferential
+ *            attachment on in-degree); rows are sorted, deduplicated, self
+ *            loops dropped. */
+hb_status hb_graph_generate_uniform(uint64_t n, uint32_t k, uint64_t seed, hb_graph *out);
+hb_status hb_graph_generate_web(uint64_t n, double alpha, uint32_t dmin, uint32_t dmax,
+                                uint64_t window, double p_window, uint64_t seed, hb_graph *out);
+void hb_graph_free(hb_graph *g);
+
+/* ---- HyperLogLog parameters (hyper_log_log_logic.rs:275-303) ------------- */
+uint32_t hb_register_size(uint64_t n);       /* HyperLogLog::register_size */
+double hb_rel_std(uint32_t log2m);            /* HyperLogLog::rel_std */
+uint32_t hb_log2_num_registers(double rsd);  /* HyperLogLog::log_2_num_of_registers */
+
+/* ---- configuration (HyperBallBuilder, hyperball_impl.rs:26-304) ---------- */
+#define HB_HASH_SIPHASH13 0 /* std DefaultHasher (SipHash-1-3, zero keys): the default of
+                               with_hyper_log_log, hyperball_impl.rs:69-71 */
+#define HB_HASH_JENKINS 1   /* the Jenkins hasher of tests/test_hyperball.rs:26-172 */
+
+#define HB_ACC_F64 0        /* per-node f64 accumulators, as hyperball_impl.rs:1072-1093 */
+#define HB_ACC_F32_KAHAN 1  /* f32 (sum, compensation) pairs */
+#define HB_ACC_F32 2        /* plain f32 sums (precision study only) */
+
+typedef struct {
+  uint32_t log2m;        /* log2 of registers per counter (with_hyper_log_log `log2m`) */
+  uint32_t reg_bits;     /* 0: HyperLogLog::register_size(num_elements); else 5 or 6 */
+  uint32_t hash;         /* HB_HASH_* */
+  uint64_t seed;         /* Jenkins seed */
+  uint64_t num_elements; /* 0: n, or the sum of the weights when weights are given */
+  const uint64_t *weights;   /* nullable: HyperBallBuilder::weights (:281-287) */
+  uint64_t weight_seed;  /* seeds the stand-in for rand::random in weighted init */
+  uint64_t arc_granularity; /* HyperBallBuilder::arc_granularity (:271-274); accepted, unused */
+  int sum_of_distances;  /* HyperBallBuilder::sum_of_distances (:259-262) */
+  int sum_of_inverse_distances; /* HyperBallBuilder::sum_of_inverse_distances (:265-268) */
+  uint32_t acc_precision; /* HB_ACC_* */
+  int device;            /* HIP device ordinal */
+} hb_config;
+
+/* Fills the defaults of HyperBallBuilder::with_hyper_log_log (:91-133):
+ * log2m 4 is NOT a default there, so log2m is set to 6; SipHash-1-3; no
+ * outputs requested; f64 accumulators; device 0. */
+void hb_config_init(hb_config *cfg);
+
+typedef struct hb_instance hb_instance;
+
+/* HyperBallBuilder::with_hyper_log_log(graph, transposed, cumul_outdeg, log2m,
+ * weights, mmap) + .sum_of_distances(..) + .sum_of_inverse_distances(..) +
+ * .build(pl)  (hyperball_impl.rs:91-133, :321-403).  `transpose` may be NULL
+ * (no systolic iterations).  Errors: HB_E_INVALID_ARG for log2m < 4, counters
+ * that are not whole 64-bit words, wrong weight/transposed sizes, or shapes
+ * this build does not hold (log2m above 12, register width other than 5 or
+ * 6). */
+hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_config *cfg,
+                    hb_instance **out);
+hb_status hb_destroy(hb_instance *hb);
+
+/* HyperBallBuilder::discount_function (:291-297).  The reference calls the
+ * closure only as f(iteration + 1) (:1089); here it is evaluated once per
+ * iteration on the host. */
+typedef double (*hb_discount_fn)(uint64_t distance, void *ctx);
+hb_status hb_add_discount_function(hb_instance *hb, hb_discount_fn f, void *ctx);
+/* HyperBallBuilder::no_discount_function (:300-303) */
+hb_status hb_clear_discount_functions(hb_instance *hb);
+
+/* ---- running (hyperball_impl.rs:534-621) --------------------------------- */
+/* HyperBall::run(upper_bound, threshold): threshold NaN means None. */
+hb_status hb_run(hb_instance *hb, uint64_t upper_bound, double threshold);
+hb_status hb_run_until_stable(hb_instance *hb, uint64_t upper_bound);
+hb_status hb_run_until_done(hb_instance *hb);
+
+/* Step-wise driving for parity checks: HyperBall::init (:1155-1213) and one
+ * HyperBall::iterate (:775-949). */
+hb_status hb_init(hb_instance *hb);
+hb_status hb_iterate(hb_instance *hb);
+
+/* ---- results (hyperball_impl.rs:633-757) --------------------------------- */
+hb_status hb_neighborhood_function(hb_instance *hb, double *out, size_t cap, size_t *len);
+hb_status hb_sum_of_distances(hb_instance *hb, double *out, size_t n);
+hb_status hb_harmonic_centralities(hb_instance *hb, double *out, size_t n);
+hb_status hb_discounted_centrality(hb_instance *hb, size_t index, double *out, size_t n);
+hb_status hb_closeness_centrality(hb_instance *hb, double *out, size_t n);
+hb_status hb_lin_centrality(hb_instance *hb, double *out, size_t n);
+hb_status hb_nieminen_centrality(hb_instance *hb, double *out, size_t n);
+hb_status hb_reachable_nodes(hb_instance *hb, double *out, size_t n);
+hb_status hb_reachable_nodes_from(hb_instance *hb, uint64_t node, double *out);
+
+/* ---- inspection ---------------------------------------------------------- */
+typedef struct {
+  uint32_t log2m, num_registers, register_bits, words_per_counter;
+  uint64_t num_nodes, num_arcs;
+  double alpha_m_m;
+} hb_params;
+hb_status hb_get_params(hb_instance *hb, hb_params *out);
+
+/* Counter arrays in the reference layout (slice_counter_array.rs:166-169,
+ * hyper_log_log_logic.rs:72-115): counter v is words [v*W, (v+1)*W), register
+ * j at bit j*r, LSB first.  which = 0: current state, 1: the other buffer. */
+hb_status hb_get_counters(hb_instance *hb, int which, uint64_t *out, size_t words);
+/* Modified-counter bits of the last iteration (AtomicBitVec layout: bit v of
+ * word v/64). */
+hb_status hb_get_modified(hb_instance *hb, uint64_t *out, size_t words);
+
+typedef struct {
+  uint64_t iteration;     /* 0-based */
+  uint32_t systolic, local, pre_local;
+  uint64_t checked;       /* nodes whose successors were enumerated (K_t) */
+  uint64_t visited_arcs;  /* merges performed (V_t), hyperball_impl.rs:1038-1039 */
+  uint64_t scanned_arcs;  /* successor ids read (A_t) */
+  uint64_t transpose_arcs;/* transpose ids read (T_t) */
+  uint64_t copied;        /* counters copied without check (Y_t) */
+  uint64_t modified;      /* modified counters (M_t) */
+  double nf;              /* neighbourhood-function value pushed */
+  double union_ms;        /* device time of the union kernel(s), when timing is on */
+  double iteration_ms;    /* device time of the whole iteration, when timing is on */
+} hb_iteration_stats;
+hb_status hb_num_iterations(hb_instance *hb, uint64_t *out);
+hb_status hb_get_iteration_stats(hb_instance *hb, uint64_t iteration, hb_iteration_stats *out);
+/* Turns per-kernel HIP-event timing on or off (off by default). */
+hb_status hb_set_timing(hb_instance *hb, int enabled);
+/* Overrides the lane-group split of the union kernel (0 restores the
+ * default); for the autotuner.  heavy_degree: rows with more successors than
+ * this go to the workgroup-per-node kernel. */
+hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

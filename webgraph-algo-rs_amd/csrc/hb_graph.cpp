@@ -1,0 +1,506 @@
+// Host-side graph input of the HyperBall library: BVGraph decoding, CSR
+// transposition and the synthetic generators.  No device code here.
+//
+// BVGraph: the reference obtains successors through the external crate
+// webgraph 0.2.0 (BvGraph::with_basename(..).load(), tests/test_hyperball.rs:
+// 226-227; successor iteration at src/algo/hyperball/hyperball_impl.rs:1037).
+// This decoder follows the published BVGraph format (Boldi & Vigna, "The
+// WebGraph Framework I", WWW 2004; format version 0): per node, the outdegree,
+// a reference into a window of previous nodes with copy blocks, intervals of
+// consecutive successors and zeta_k-coded residual gaps, all in MSB-first bit
+// streams.
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+
+#include "hb_common.hpp"
+
+namespace hb {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+// ------------------------------------------------------------------------
+// Bit input, MSB first (dsiutils InputBitStream conventions).
+// ------------------------------------------------------------------------
+class BitReader {
+ public:
+  explicit BitReader(std::vector<uint8_t> bytes) : buf_(std::move(bytes)) {
+    nbits_ = uint64_t(buf_.size()) * 8;
+    buf_.resize(buf_.size() + 16, 0);  // padding for 8-byte window loads
+  }
+  uint64_t position() const { return pos_; }
+  void seek(uint64_t bit) { pos_ = bit; }
+
+  // 64-bit window starting at the current position.
+  uint64_t window() const {
+    uint64_t byte = pos_ >> 3;
+    uint64_t w = 0;
+    for (int k = 0; k < 8; k++) w = (w << 8) | buf_[byte + k];
+    unsigned sh = pos_ & 7;
+    if (sh) w = (w << sh) | (uint64_t(buf_[byte + 8]) >> (8 - sh));
+    return w;
+  }
+  uint64_t read_int(unsigned len) {  // len <= 64
+    if (len == 0) return 0;
+    check(len);
+    uint64_t v = window() >> (64 - len);
+    pos_ += len;
+    return v;
+  }
+  uint64_t read_bit() { return read_int(1); }
+  uint64_t read_unary() {
+    uint64_t zeros = 0;
+    for (;;) {
+      check(1);
+      uint64_t w = window();
+      if (w) {
+        unsigned lz = __builtin_clzll(w);
+        zeros += lz;
+        pos_ += lz + 1;
+        return zeros;
+      }
+      zeros += 64;
+      pos_ += 64;
+    }
+  }
+  uint64_t read_gamma() {
+    uint64_t msb = read_unary();
+    if (msb >= 64) fail(HB_E_INVALID_ARG, "BVGraph: gamma code out of range");
+    return ((uint64_t(1) << msb) | read_int(unsigned(msb))) - 1;
+  }
+  uint64_t read_delta() {
+    uint64_t msb = read_gamma();
+    if (msb >= 64) fail(HB_E_INVALID_ARG, "BVGraph: delta code out of range");
+    return ((uint64_t(1) << msb) | read_int(unsigned(msb))) - 1;
+  }
+  // zeta_k: h in unary, then a minimal binary code of x + 1 - 2^(hk) over
+  // [0, 2^((h+1)k) - 2^(hk)).
+  uint64_t read_zeta(unsigned k) {
+    if (k == 1) return read_gamma();
+    uint64_t h = read_unary();
+    if ((h + 1) * k > 63) fail(HB_E_INVALID_ARG, "BVGraph: zeta code out of range");
+    uint64_t left = uint64_t(1) << (h * k);
+    uint64_t m = read_int(unsigned(h * k + k - 1));
+    if (m < left) return m + left - 1;
+    return (m << 1) + read_bit() - 1;
+  }
+
+ private:
+  void check(unsigned len) const {
+    if (pos_ + len > nbits_) fail(HB_E_INVALID_ARG, "BVGraph: read past the end of the bit stream");
+  }
+  std::vector<uint8_t> buf_;
+  uint64_t nbits_ = 0, pos_ = 0;
+};
+
+enum Code { GAMMA, DELTA, UNARY, ZETA };
+
+struct BvProps {
+  uint64_t nodes = 0, arcs = 0;
+  unsigned window = 7, min_interval = 4, zeta_k = 3;
+  int version = 0;
+  Code outdegrees = GAMMA, references = UNARY, blocks = GAMMA, block_count = GAMMA,
+       residuals = ZETA, offsets = GAMMA;
+};
+
+std::vector<uint8_t> read_file(const std::string &path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) fail(HB_E_INVALID_ARG, "cannot open " + path);
+  f.seekg(0, std::ios::end);
+  std::streamoff sz = f.tellg();
+  f.seekg(0);
+  std::vector<uint8_t> v(size_t(sz > 0 ? sz : 0));
+  if (sz > 0) f.read(reinterpret_cast<char *>(v.data()), sz);
+  return v;
+}
+
+Code parse_code(const std::string &name, const std::string &flag) {
+  if (name == "GAMMA") return GAMMA;
+  if (name == "DELTA") return DELTA;
+  if (name == "UNARY") return UNARY;
+  if (name == "ZETA") return ZETA;
+  fail(HB_E_INVALID_ARG, "BVGraph: unsupported compression flag " + flag);
+}
+
+BvProps read_properties(const std::string &basename) {
+  std::ifstream f(basename + ".properties");
+  if (!f) fail(HB_E_INVALID_ARG, "cannot open " + basename + ".properties");
+  std::map<std::string, std::string> kv;
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    auto eq = line.find('=');
+    if (eq == std::string::npos) continue;
+    std::string val = line.substr(eq + 1);
+    while (!val.empty() && (val.back() == '\r' || val.back() == ' ')) val.pop_back();
+    kv[line.substr(0, eq)] = val;
+  }
+  BvProps p;
+  auto num = [&](const char *k, uint64_t def) -> uint64_t {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : std::stoull(it->second);
+  };
+  if (!kv.count("nodes") || !kv.count("arcs"))
+    fail(HB_E_INVALID_ARG, "BVGraph: properties lack nodes/arcs");
+  p.nodes = num("nodes", 0);
+  p.arcs = num("arcs", 0);
+  p.window = unsigned(num("windowsize", 7));
+  p.min_interval = unsigned(num("minintervallength", 4));
+  p.zeta_k = unsigned(num("zetak", 3));
+  p.version = int(num("version", 0));
+  if (p.version != 0) fail(HB_E_INVALID_ARG, "BVGraph: only format version 0 is supported");
+  if (kv.count("endianness") && kv["endianness"] != "big")
+    fail(HB_E_INVALID_ARG, "BVGraph: only big-endian bit streams are supported");
+  if (p.nodes > 0xffffffffull) fail(HB_E_INVALID_ARG, "BVGraph: more than 2^32 nodes");
+  std::string flags = kv.count("compressionflags") ? kv["compressionflags"] : "";
+  std::stringstream ss(flags);
+  std::string tok;
+  while (std::getline(ss, tok, '|')) {
+    while (!tok.empty() && tok.front() == ' ') tok.erase(tok.begin());
+    while (!tok.empty() && tok.back() == ' ') tok.pop_back();
+    if (tok.empty()) continue;
+    auto us = tok.rfind('_');
+    if (us == std::string::npos) fail(HB_E_INVALID_ARG, "BVGraph: bad compression flag " + tok);
+    std::string comp = tok.substr(0, us), code = tok.substr(us + 1);
+    Code c = parse_code(code, tok);
+    if (comp == "OUTDEGREES") p.outdegrees = c;
+    else if (comp == "REFERENCES") p.references = c;
+    else if (comp == "BLOCKS") p.blocks = c;
+    else if (comp == "BLOCK_COUNT") p.block_count = c;
+    else if (comp == "RESIDUALS") p.residuals = c;
+    else if (comp == "OFFSETS") p.offsets = c;
+    else fail(HB_E_INVALID_ARG, "BVGraph: bad compression flag " + tok);
+  }
+  return p;
+}
+
+uint64_t read_code(BitReader &in, Code c, unsigned zk) {
+  switch (c) {
+    case GAMMA: return in.read_gamma();
+    case DELTA: return in.read_delta();
+    case UNARY: return in.read_unary();
+    case ZETA: return in.read_zeta(zk);
+  }
+  return 0;
+}
+
+// int2nat inverse: 0,1,2,3,4 -> 0,-1,1,-2,2.
+int64_t nat2int(uint64_t x) { return int64_t(x >> 1) ^ -int64_t(x & 1); }
+
+// Sequential decode of every successor list.
+void decode_bvgraph(const std::string &basename, std::vector<uint64_t> &offsets,
+                    std::vector<uint32_t> &succ, std::vector<uint64_t> *bit_offsets) {
+  BvProps p = read_properties(basename);
+  BitReader in(read_file(basename + ".graph"));
+  const uint64_t n = p.nodes;
+  offsets.assign(n + 1, 0);
+  succ.clear();
+  succ.reserve(p.arcs);
+  if (bit_offsets) bit_offsets->assign(n + 1, 0);
+  std::vector<int64_t> extra, blocks;
+  std::vector<uint32_t> merged;
+  for (uint64_t x = 0; x < n; x++) {
+    if (bit_offsets) (*bit_offsets)[x] = in.position();
+    offsets[x] = succ.size();
+    uint64_t d = read_code(in, p.outdegrees, p.zeta_k);
+    if (d == 0) continue;
+    merged.clear();
+    uint64_t ref = p.window > 0 ? read_code(in, p.references, p.zeta_k) : 0;
+    if (ref > x) fail(HB_E_INVALID_ARG, "BVGraph: reference before node 0");
+    uint64_t copied_total = 0;
+    if (ref > 0) {
+      uint64_t rnode = x - ref;
+      uint64_t rb = offsets[rnode], re = offsets[rnode + 1];
+      uint64_t rdeg = re - rb;
+      uint64_t bc = read_code(in, p.block_count, p.zeta_k);
+      blocks.assign(bc, 0);
+      for (uint64_t i = 0; i < bc; i++)
+        blocks[i] = int64_t(read_code(in, p.blocks, p.zeta_k)) + (i == 0 ? 0 : 1);
+      // Blocks alternate copy / skip, starting with copy; after the last
+      // block the rest of the reference list is copied iff the count is even.
+      uint64_t pos = 0;
+      for (uint64_t i = 0; i < bc; i++) {
+        uint64_t len = uint64_t(blocks[i]);
+        if (pos + len > rdeg) fail(HB_E_INVALID_ARG, "BVGraph: copy block past reference list");
+        if ((i & 1) == 0)
+          for (uint64_t k = 0; k < len; k++) merged.push_back(succ[rb + pos + k]);
+        pos += len;
+      }
+      if ((bc & 1) == 0)
+        for (uint64_t k = pos; k < rdeg; k++) merged.push_back(succ[rb + k]);
+      copied_total = merged.size();
+    }
+    if (copied_total > d) fail(HB_E_INVALID_ARG, "BVGraph: more copied successors than outdegree");
+    uint64_t extra_count = d - copied_total;
+    if (extra_count > 0 && p.min_interval > 0) {
+      uint64_t ic = in.read_gamma();
+      int64_t prev = 0;
+      for (uint64_t i = 0; i < ic; i++) {
+        int64_t left;
+        if (i == 0) left = nat2int(in.read_gamma()) + int64_t(x);
+        else left = int64_t(in.read_gamma()) + prev + 1;
+        uint64_t len = in.read_gamma() + p.min_interval;
+        if (left < 0 || uint64_t(left) + len > n || len > extra_count)
+          fail(HB_E_INVALID_ARG, "BVGraph: interval out of range");
+        for (uint64_t k = 0; k < len; k++) merged.push_back(uint32_t(left + int64_t(k)));
+        prev = left + int64_t(len);
+        extra_count -= len;
+      }
+    }
+    int64_t prev = 0;
+    for (uint64_t i = 0; i < extra_count; i++) {
+      int64_t s;
+      if (i == 0) s = int64_t(x) + nat2int(read_code(in, p.residuals, p.zeta_k));
+      else s = prev + int64_t(read_code(in, p.residuals, p.zeta_k)) + 1;
+      if (s < 0 || uint64_t(s) >= n) fail(HB_E_INVALID_ARG, "BVGraph: residual out of range");
+      merged.push_back(uint32_t(s));
+      prev = s;
+    }
+    std::sort(merged.begin(), merged.end());
+    succ.insert(succ.end(), merged.begin(), merged.end());
+  }
+  offsets[n] = succ.size();
+  if (bit_offsets) (*bit_offsets)[n] = in.position();
+  if (succ.size() != p.arcs)
+    fail(HB_E_INVALID_ARG, "BVGraph: decoded " + std::to_string(succ.size()) +
+                               " arcs, properties say " + std::to_string(p.arcs));
+}
+
+template <class T>
+T *copy_out(const std::vector<T> &v) {
+  T *p = static_cast<T *>(std::malloc(std::max<size_t>(1, v.size()) * sizeof(T)));
+  if (!p) throw std::bad_alloc();
+  if (!v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(T));
+  return p;
+}
+
+// ------------------------------------------------------------------------
+// Random numbers: xoshiro256** (Blackman & Vigna 2018) seeded by SplitMix64.
+// ------------------------------------------------------------------------
+inline uint64_t splitmix_next(uint64_t &z) {
+  uint64_t r = (z += 0x9e3779b97f4a7c15ull);
+  r = (r ^ (r >> 30)) * 0xbf58476d1ce4e5b9ull;
+  r = (r ^ (r >> 27)) * 0x94d049bb133111ebull;
+  return r ^ (r >> 31);
+}
+inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+struct Rng {
+  uint64_t s[4];
+  Rng(uint64_t seed, uint64_t stream) {
+    uint64_t z = seed ^ (stream * 0xd1b54a32d192ed03ull);
+    for (auto &w : s) w = splitmix_next(z);
+  }
+  uint64_t next() {
+    uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+    s[2] ^= t; s[3] = rotl(s[3], 45);
+    return r;
+  }
+  uint64_t below(uint64_t n) { return uint64_t((unsigned __int128)next() * n >> 64); }
+  double uniform() { return double(next() >> 11) * 0x1p-53; }
+};
+
+void fill_graph(hb_graph *out, uint64_t n, std::vector<uint64_t> &&off, std::vector<uint32_t> &&succ) {
+  out->num_nodes = n;
+  out->num_arcs = succ.size();
+  out->offsets = copy_out(off);
+  out->successors = copy_out(succ);
+}
+
+}  // namespace
+}  // namespace hb
+
+using namespace hb;
+
+extern "C" {
+
+const char *hb_last_error(void) { return hb::g_last_error.c_str(); }
+
+hb_status hb_bvgraph_load(const char *basename, hb_graph *out) {
+  return guard([&] {
+    if (!basename || !out) fail(HB_E_INVALID_ARG, "null argument");
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> succ;
+    decode_bvgraph(basename, off, succ, nullptr);
+    fill_graph(out, off.size() - 1, std::move(off), std::move(succ));
+  });
+}
+
+hb_status hb_bvgraph_decoder_offsets(const char *basename, uint64_t *out, uint64_t cap) {
+  return guard([&] {
+    if (!basename || !out) fail(HB_E_INVALID_ARG, "null argument");
+    std::vector<uint64_t> off, bits;
+    std::vector<uint32_t> succ;
+    decode_bvgraph(basename, off, succ, &bits);
+    if (cap < bits.size()) fail(HB_E_INVALID_ARG, "output buffer too small");
+    std::memcpy(out, bits.data(), bits.size() * sizeof(uint64_t));
+  });
+}
+
+hb_status hb_bvgraph_load_offsets(const char *basename, uint64_t num_nodes, uint64_t *out) {
+  return guard([&] {
+    if (!basename || !out) fail(HB_E_INVALID_ARG, "null argument");
+    BvProps p = read_properties(basename);
+    BitReader in(read_file(std::string(basename) + ".offsets"));
+    uint64_t acc = 0;
+    for (uint64_t i = 0; i <= num_nodes; i++) {
+      acc += read_code(in, p.offsets, p.zeta_k);
+      out[i] = acc;
+    }
+  });
+}
+
+hb_status hb_graph_transpose(const hb_csr *g, hb_graph *out) {
+  return guard([&] {
+    if (!g || !out || !g->offsets || (g->num_arcs && !g->successors))
+      fail(HB_E_INVALID_ARG, "null argument");
+    const uint64_t n = g->num_nodes, a = g->num_arcs;
+    std::vector<uint64_t> toff(n + 1, 0);
+    std::vector<uint32_t> tsucc(a);
+    std::unique_ptr<std::atomic<uint64_t>[]> cnt(new std::atomic<uint64_t>[n + 1]);
+    for (uint64_t i = 0; i <= n; i++) cnt[i].store(0, std::memory_order_relaxed);
+    parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t u = b; u < e; u++)
+        for (uint64_t k = g->offsets[u]; k < g->offsets[u + 1]; k++) {
+          uint32_t v = g->successors[k];
+          if (v >= n) fail(HB_E_INVALID_ARG, "successor out of range");
+          cnt[v].fetch_add(1, std::memory_order_relaxed);
+        }
+    });
+    for (uint64_t v = 0; v < n; v++) toff[v + 1] = toff[v] + cnt[v].load();
+    for (uint64_t v = 0; v < n; v++) cnt[v].store(toff[v], std::memory_order_relaxed);
+    parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t u = b; u < e; u++)
+        for (uint64_t k = g->offsets[u]; k < g->offsets[u + 1]; k++)
+          tsucc[cnt[g->successors[k]].fetch_add(1, std::memory_order_relaxed)] = uint32_t(u);
+    });
+    parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t v = b; v < e; v++) std::sort(tsucc.begin() + toff[v], tsucc.begin() + toff[v + 1]);
+    });
+    fill_graph(out, n, std::move(toff), std::move(tsucc));
+  });
+}
+
+hb_status hb_graph_generate_uniform(uint64_t n, uint32_t k, uint64_t seed, hb_graph *out) {
+  return guard([&] {
+    if (!out) fail(HB_E_INVALID_ARG, "null argument");
+    if (n > 0xffffffffull) fail(HB_E_INVALID_ARG, "more than 2^32 nodes");
+    if (n > 0 && uint64_t(k) > n - 1) fail(HB_E_INVALID_ARG, "k exceeds n - 1");
+    std::vector<uint64_t> off(n + 1);
+    for (uint64_t v = 0; v <= n; v++) off[v] = v * k;
+    std::vector<uint32_t> succ(n * k);
+    parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+      std::vector<uint64_t> s;
+      for (uint64_t v = b; v < e; v++) {
+        Rng rng(seed, v);
+        s.clear();
+        // Floyd's sampling of k distinct values from [0, n-1), then skip v.
+        const uint64_t N = n - 1;
+        for (uint64_t j = N - k; j < N; j++) {
+          uint64_t t = rng.below(j + 1);
+          if (std::find(s.begin(), s.end(), t) != s.end()) s.push_back(j);
+          else s.push_back(t);
+        }
+        for (auto &x : s) x += (x >= v);
+        std::sort(s.begin(), s.end());
+        for (uint32_t i = 0; i < k; i++) succ[v * k + i] = uint32_t(s[i]);
+      }
+    });
+    fill_graph(out, n, std::move(off), std::move(succ));
+  });
+}
+
+hb_status hb_graph_generate_web(uint64_t n, double alpha, uint32_t dmin, uint32_t dmax,
+                                uint64_t window, double p_window, uint64_t seed, hb_graph *out) {
+  return guard([&] {
+    if (!out) fail(HB_E_INVALID_ARG, "null argument");
+    if (n < 2 || n > 0xffffffffull) fail(HB_E_INVALID_ARG, "n must lie in [2, 2^32]");
+    if (dmin < 1 || dmax < dmin) fail(HB_E_INVALID_ARG, "bad degree range");
+    if (window < 1) fail(HB_E_INVALID_ARG, "window must be positive");
+    // Zipf(alpha) on [dmin, dmax] by inverse CDF.
+    std::vector<double> cdf(dmax - dmin + 1);
+    double acc = 0;
+    for (uint32_t d = dmin; d <= dmax; d++) cdf[d - dmin] = (acc += std::pow(double(d), -alpha));
+    for (auto &c : cdf) c /= acc;
+    std::vector<uint64_t> deg(n);
+    parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t v = b; v < e; v++) {
+        Rng rng(seed, v);
+        double u = rng.uniform();
+        uint64_t idx = uint64_t(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+        deg[v] = dmin + std::min<uint64_t>(idx, dmax - dmin);
+      }
+    });
+    std::vector<uint64_t> slot(n + 1, 0);
+    for (uint64_t v = 0; v < n; v++) slot[v + 1] = slot[v] + deg[v];
+    const uint64_t total = slot[n];
+    // Draw every arc slot.  High bit set: copy the target of an earlier slot.
+    const uint64_t COPY = uint64_t(1) << 63;
+    std::vector<uint64_t> draw(total);
+    parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t v = b; v < e; v++) {
+        Rng rng(seed, v);
+        rng.uniform();  // the degree draw
+        for (uint64_t k = slot[v]; k < slot[v + 1]; k++) {
+          if (rng.uniform() < p_window || k == 0) {
+            uint64_t off = rng.below(2 * window) ;  // [0, 2w) -> [-w, -1] U [1, w]
+            int64_t delta = off < window ? -int64_t(window - off) : int64_t(off - window + 1);
+            int64_t t = (int64_t(v) + delta) % int64_t(n);
+            if (t < 0) t += int64_t(n);
+            draw[k] = uint64_t(t);
+          } else {
+            draw[k] = COPY | rng.below(k);
+          }
+        }
+      }
+    });
+    std::vector<uint32_t> target(total);
+    parallel_for(total, [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t k = b; k < e; k++) {
+        uint64_t j = k;
+        while (draw[j] & COPY) j = draw[j] & ~COPY;
+        target[k] = uint32_t(draw[j]);
+      }
+    });
+    std::vector<uint64_t>().swap(draw);
+    // Sort, deduplicate and drop self loops per row.
+    std::vector<uint64_t> keep(n, 0);
+    parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t v = b; v < e; v++) {
+        auto first = target.begin() + slot[v], last = target.begin() + slot[v + 1];
+        std::sort(first, last);
+        auto end = std::unique(first, last);
+        end = std::remove(first, end, uint32_t(v));
+        keep[v] = uint64_t(end - first);
+      }
+    });
+    std::vector<uint64_t> off(n + 1, 0);
+    for (uint64_t v = 0; v < n; v++) off[v + 1] = off[v] + keep[v];
+    std::vector<uint32_t> succ(off[n]);
+    parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+      for (uint64_t v = b; v < e; v++)
+        std::copy(target.begin() + slot[v], target.begin() + slot[v] + keep[v], succ.begin() + off[v]);
+    });
+    fill_graph(out, n, std::move(off), std::move(succ));
+  });
+}
+
+void hb_graph_free(hb_graph *g) {
+  if (!g) return;
+  std::free(g->offsets);
+  std::free(g->successors);
+  g->offsets = nullptr;
+  g->successors = nullptr;
+  g->num_nodes = g->num_arcs = 0;
+}
+
+}  // extern "C"
