@@ -310,7 +310,8 @@ void or_merge_max(const or_hll *h, uint64_t *x, const uint64_t *y) {
 /* ---------------------------------------------------------------------------
  * Sequential Jacobi HyperBall: hyperball_impl.rs:1230-1261 (SeqHyperBall).
  * next(v) = curr(v) merged with curr(s) for every successor s, self loops
- * included (they change nothing).
+ * included (they change nothing).  The merge is the reference's broadword
+ * merge, which the tests pin to or_merge_max.
  * ------------------------------------------------------------------------- */
 
 static void seed_counters(const or_hll *h, uint64_t n, uint64_t *curr, const uint64_t *weights,
@@ -343,7 +344,7 @@ uint64_t or_jacobi_iterate(const or_hll *h, uint64_t n, const uint64_t *offsets,
     uint64_t *dst = next + v * W;
     memcpy(dst, curr + v * W, W * sizeof(uint64_t));
     for (uint64_t k = offsets[v]; k < offsets[v + 1]; k++)
-      or_merge_max(h, dst, curr + (uint64_t)succ[k] * W);
+      or_merge_broadword(h, dst, curr + (uint64_t)succ[k] * W);
     if (memcmp(dst, curr + v * W, W * sizeof(uint64_t)) != 0) {
 #pragma omp atomic
       modified_bits[v / 64] |= 1ull << (v % 64);

@@ -1,0 +1,126 @@
+"""GPU parity: the HIP engine against the CPU oracle, through the C ABI.
+
+Registers (in the reference layout) and modified bits must be bit-exact with
+the sequential Jacobi oracle after every iteration
+(src/algo/hyperball/hyperball_impl.rs:1302-1319 in the reference); f64
+accumulators must be bit-exact with the oracle's restatement of the reference
+engine; the neighbourhood function agrees within 1e-12 relative (the
+reference's own value depends on thread order).
+"""
+import numpy as np
+import pytest
+
+from golden_io import assert_array_equal_rel, cnr_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _tiny_graphs(hbm):
+    n = 40
+    path = hbm.Graph.from_arcs(n, [(i, i + 1) for i in range(n - 1)])
+    cycle = hbm.Graph.from_arcs(n, [(i, (i + 1) % n) for i in range(n)])
+    star = hbm.Graph.from_arcs(n, [(0, i) for i in range(1, n)] + [(i, 0) for i in range(1, 5)])
+    clique = hbm.Graph.from_arcs(12, [(i, j) for i in range(12) for j in range(12) if i != j])
+    empty = hbm.Graph.from_arcs(10, [])
+    loops = hbm.Graph.from_arcs(10, [(i, i) for i in range(10)] + [(1, 2), (2, 2)])
+    return {"path": path, "cycle": cycle, "star": star, "clique": clique, "empty": empty,
+            "self_loops": loops}
+
+
+def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_000):
+    b = hbm.HyperBallBuilder.with_hyper_log_log(g, gt, log2m).register_bits(reg_bits)
+    if jenkins is not None:
+        b = b.jenkins(jenkins)
+    hb = b.build()
+    hll = orc.Hll(log2m, reg_bits, g.num_nodes, orc.HASH_JENKINS if jenkins is not None else orc.HASH_SIPHASH13,
+                  jenkins or 0)
+    ref = orc.Jacobi(hll, g.offsets, g.successors)
+    hb.init()
+    assert np.array_equal(hb.counters(0), ref.curr), "seeded registers differ"
+    it = 0
+    while it < max_iter:
+        hb.iterate()
+        k = ref.iterate()
+        it += 1
+        assert np.array_equal(hb.counters(0), ref.curr), f"registers differ after iteration {it}"
+        assert np.array_equal(hb.counters(1), ref.next), f"previous registers differ after iteration {it}"
+        assert np.array_equal(hb.modified_bits(), ref.modified), f"modified bits differ after iteration {it}"
+        if k == 0:
+            break
+    stats = hb.iteration_stats()
+    hb.close()
+    return it, stats
+
+
+@pytest.mark.parametrize("name", ["path", "cycle", "star", "clique", "empty", "self_loops"])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_tiny_graphs_bit_exact(hbm, orc, name, transpose):
+    g = _tiny_graphs(hbm)[name]
+    gt = g.transpose() if transpose else None
+    _run_lockstep(hbm, orc, g, gt, 6, jenkins=42)
+
+
+@pytest.mark.parametrize("log2m,reg_bits", [(5, 6), (6, 5), (7, 5), (8, 5), (9, 6), (10, 5), (10, 6),
+                                            (11, 5), (12, 5), (12, 6)])
+def test_uniform_all_shapes_bit_exact(hbm, orc, log2m, reg_bits):
+    g = hbm.Graph.uniform(3000, 3, seed=11)
+    _run_lockstep(hbm, orc, g, g.transpose(), log2m, reg_bits, jenkins=7)
+
+
+@pytest.mark.parametrize("hash_", ["siphash", "jenkins"])
+def test_web_systolic_local_bit_exact(hbm, orc, hash_):
+    g = hbm.Graph.web(20000, dmin=1, dmax=200, window=100, seed=5)
+    gt = g.transpose()
+    it, stats = _run_lockstep(hbm, orc, g, gt, 8, jenkins=(None if hash_ == "siphash" else 42))
+    modes = {(s["systolic"], s["local"], s["pre_local"]) for s in stats}
+    assert any(m[0] for m in modes), "no systolic iteration was exercised"
+
+
+def test_engine_outputs_match_reference_engine(hbm, orc):
+    g = hbm.Graph.web(30000, dmin=1, dmax=300, window=200, seed=9)
+    gt = g.transpose()
+    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).jenkins(3)
+          .sum_of_distances(True).sum_of_inverse_distances(True).build())
+    hb.run_until_done()
+    ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=8,
+                              hash=orc.HASH_JENKINS, seed=3, sum_of_distances=True,
+                              sum_of_inverse_distances=True)
+    ref.run()
+    assert np.array_equal(hb.counters(0), ref.counters(0))
+    # per-node f64 accumulators are deterministic in the reference: bit-exact
+    assert np.array_equal(hb.sum_of_distances(), ref.sum_of_distances())
+    assert np.array_equal(hb.harmonic_centralities(), ref.harmonic_centralities())
+    assert np.array_equal(hb.lin_centrality(), ref.lin_centrality())
+    assert np.array_equal(hb.closeness_centrality(), ref.closeness_centrality())
+    assert np.array_equal(hb.nieminen_centrality(), ref.nieminen_centrality())
+    assert np.array_equal(hb.reachable_nodes(), ref.reachable_nodes())
+    nf, nf_ref = hb.neighborhood_function(), ref.neighborhood_function()
+    assert len(nf) == len(nf_ref)
+    np.testing.assert_allclose(nf, nf_ref, rtol=1e-12, atol=0)
+
+
+def test_cnr_2000_java_goldens(hbm, cnr):
+    """tests/test_hyperball.rs:221-302 of the reference: Jenkins seed 42,
+    log2m 8, sum of distances and harmonic, run until done, 1e-6 relative."""
+    g, gt = cnr
+    hb = (hbm.HyperBallBuilder.with_transpose(g, gt, 8).jenkins(42)
+          .sum_of_distances(True).sum_of_inverse_distances(True).build())
+    hb.run_until_done()
+    threshold = 1e-6
+    assert_array_equal_rel(cnr_golden("sum_of_distances"), hb.sum_of_distances(), threshold, "sum of distances")
+    assert_array_equal_rel(cnr_golden("harmonic_centrality"), hb.harmonic_centralities(), threshold,
+                           "harmonic centralities")
+    assert_array_equal_rel(cnr_golden("lin_centrality"), hb.lin_centrality(), threshold, "lin centralities")
+    assert_array_equal_rel(cnr_golden("closeness_centrality"), hb.closeness_centrality(), threshold,
+                           "closeness centralities")
+    assert_array_equal_rel(cnr_golden("nieminen_centrality"), hb.nieminen_centrality(), threshold,
+                           "nieminen centralities")
+
+
+def test_cnr_2000_par_equals_seq_every_iteration(hbm, orc, cnr):
+    """The reference's white-box test (hyperball_impl.rs:1263-1322): log2m 6,
+    DefaultHasher, with transpose; every iteration equals sequential Jacobi."""
+    g, gt = cnr
+    it, stats = _run_lockstep(hbm, orc, g, gt, 6)
+    assert it > 10
+    assert any(s["systolic"] for s in stats) and any(s["local"] for s in stats)

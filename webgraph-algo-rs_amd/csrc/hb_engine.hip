@@ -1,0 +1,740 @@
+// HyperBall engine: host control of the device state and the C ABI of
+// include/hyperball.h.
+//
+// The per-iteration control logic restates HyperBall::iterate
+// (src/algo/hyperball/hyperball_impl.rs:775-949): the systolic / local /
+// pre-local mode choice, the neighbourhood-function bookkeeping (last value
+// kept unclamped, pushed value clamped to be monotone, relative increment),
+// and the swaps of the double-buffered state.  Work distribution differs by
+// design: instead of the reference's arc-balanced chunks and its local list,
+// a systolic iteration compacts the bitmap (must-be-checked | modified) into
+// an ordered worklist on the device and launches the union kernel over it.
+// Registers and modified bits do not depend on the schedule (the reference's
+// own test asserts its parallel run equals a sequential Jacobi run,
+// hyperball_impl.rs:1302-1319).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hb_common.hpp"
+#include "hb_hll.hpp"
+#include "hb_kernels.hpp"
+
+namespace hb {
+namespace {
+
+#define HB_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      fail(e_ == hipErrorOutOfMemory ? HB_E_OOM : HB_E_HIP,                                 \
+           std::string(#call) + " failed: " + hipGetErrorString(e_));                      \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t n) {
+    release();
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(1, n) * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(HB_E_OOM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                         hipGetErrorString(e));
+    }
+    p = static_cast<T *>(q);
+    count = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+};
+
+struct Discount {
+  hb_discount_fn f;
+  void *ctx;
+};
+
+const char *kNotRun =
+    "HyperBall was not run. Please call HyperBall::run before accessing computed fields";
+
+}  // namespace
+
+struct Instance {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t cus = 256;
+
+  uint64_t n = 0, a = 0;
+  uint32_t log2m = 0, m = 0, r = 0, W = 0, cdw = 0;
+  uint64_t sentinel = 0;
+  double amm = 0.0;
+  uint32_t hash = 0;
+  uint64_t seed = 0, weight_seed = 0;
+  bool weighted = false, has_t = false, do_sd = false, do_hc = false;
+  uint32_t acc_mode = ACC_F64;
+  std::vector<Discount> discounts;
+  uint32_t heavy_degree = 0;
+
+  DevBuf<uint64_t> off, toff, d_weights, d_wprefix;
+  DevBuf<uint32_t> succ, tsucc;
+  DevBuf<uint32_t> cnt[2], mod[2], must[2];
+  DevBuf<double> cache, lc;
+  DevBuf<uint8_t> sd, hc;
+  std::vector<std::unique_ptr<DevBuf<uint8_t>>> dc;
+  DevBuf<uint32_t> list, bsum, list_len;
+  DevBuf<Partial> partials;
+  DevBuf<Stats> stats;
+  Stats *h_stats = nullptr;
+  uint32_t max_grid = 0;
+  uint64_t bitmap_words = 0;  // u32 words, even
+
+  int cur = 0, mcur = 0, mustcur = 0;
+  bool inited = false;
+  uint64_t iteration = 0;
+  bool systolic = false, local = false, pre_local = false;
+  double last = 0.0, rel_inc = 0.0;
+  uint64_t modified = 0;
+  std::vector<double> nf;
+  std::vector<hb_iteration_stats> history;
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+  ~Instance() {
+    if (stream) {
+      (void)hipSetDevice(device);
+      (void)hipStreamSynchronize(stream);
+    }
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (h_stats) (void)hipHostFree(h_stats);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  size_t acc_bytes() const { return acc_mode == ACC_F32 ? 4 : 8; }
+  bool do_centrality() const { return do_sd || do_hc || !discounts.empty(); }
+  void use_device() const { HB_HIP(hipSetDevice(device)); }
+
+  void fill_zero(void *p, size_t bytes) {
+    HB_HIP(launch_fill_u32(static_cast<uint32_t *>(p), bytes / 4, 0u, stream));
+  }
+
+  void init() {
+    use_device();
+    for (auto &c : cnt) fill_zero(c.p, size_t(n) * cdw * 4);
+    SeedArgs s{};
+    s.counters = cnt[0].p;
+    s.n = n;
+    s.cdw = cdw;
+    s.r = r;
+    s.log2m = log2m;
+    s.sentinel = sentinel;
+    s.hash = hash;
+    s.seed = seed;
+    s.weights = weighted ? d_weights.p : nullptr;
+    s.weight_prefix = weighted ? d_wprefix.p : nullptr;
+    s.weight_seed = weight_seed;
+    HB_HIP(launch_seed(s, stream));
+    HB_HIP(launch_count_all(int(r), int(log2m), cnt[0].p, n, amm, m, lc.p, cache.p, stream));
+    if (do_sd) fill_zero(sd.p, n * acc_bytes());
+    if (do_hc) fill_zero(hc.p, n * acc_bytes());
+    for (auto &d : dc) fill_zero(d->p, n * acc_bytes());
+    HB_HIP(launch_fill_bits(mod[0].p, n, stream));
+    fill_zero(mod[1].p, bitmap_words * 4);
+    fill_zero(must[0].p, bitmap_words * 4);
+    fill_zero(must[1].p, bitmap_words * 4);
+    HB_HIP(hipStreamSynchronize(stream));
+    cur = mcur = mustcur = 0;
+    iteration = 0;
+    systolic = local = pre_local = false;
+    modified = 0;
+    last = double(n);
+    rel_inc = 0.0;
+    nf.assign(1, last);
+    history.clear();
+    inited = true;
+  }
+
+  void iterate() {
+    if (!inited) fail(HB_E_INVALID_ARG, "hb_iterate called before hb_init");
+    use_device();
+    const uint64_t modified_prev = modified;
+    const bool prev_systolic = systolic;
+    systolic = has_t && iteration > 0 && modified_prev < n / 4;
+    const double base_nf = systolic ? last : 0.0;
+    local = pre_local;
+    pre_local = systolic && a > 0 && modified_prev < (n * n) / (a * 10);
+
+    uint32_t *cmod = mod[mcur].p, *nmod = mod[1 - mcur].p;
+    uint32_t *cmust = must[mustcur].p, *nmust = must[1 - mustcur].p;
+    if (timing) HB_HIP(hipEventRecord(ev[0], stream));
+    fill_zero(nmod, bitmap_words * 4);
+    if (systolic) fill_zero(nmust, bitmap_words * 4);
+
+    uint32_t flags = 0;
+    bool list_mode = false;
+    if (systolic) flags |= F_SYSTOLIC;
+    if (do_centrality()) flags |= F_CENTRALITY;
+    if (local) {
+      list_mode = true;
+      flags |= F_CHECK_ALL;
+    } else if (systolic && prev_systolic) {
+      list_mode = true;
+    } else {
+      flags |= F_CHECK_ALL;  // standard, or the first systolic iteration (all nodes checked)
+    }
+    if (list_mode) {
+      HB_HIP(launch_compact(cmust, cmod, n, bsum.p, list.p, list_len.p, stream));
+      flags |= F_LIST;
+    }
+
+    UnionArgs ua{};
+    ua.off = off.p;
+    ua.succ = succ.p;
+    ua.toff = toff.p;
+    ua.tsucc = tsucc.p;
+    ua.curr = cnt[cur].p;
+    ua.next = cnt[1 - cur].p;
+    ua.curr_mod = cmod;
+    ua.next_mod = nmod;
+    ua.must = cmust;
+    ua.next_must = nmust;
+    ua.cache = cache.p;
+    ua.list = list.p;
+    ua.list_len = list_len.p;
+    ua.count = n;
+    ua.flags = flags;
+    ua.acc_mode = acc_mode;
+    ua.sd = do_sd ? static_cast<void *>(sd.p) : nullptr;
+    ua.hc = do_hc ? static_cast<void *>(hc.p) : nullptr;
+    ua.ndisc = int(discounts.size());
+    for (size_t k = 0; k < discounts.size(); k++) {
+      ua.dc[k] = dc[k]->p;
+      ua.disc[k] = discounts[k].f(iteration + 1, discounts[k].ctx);
+    }
+    ua.dist = double(iteration + 1);
+    ua.alpha_m_m = amm;
+    ua.m = m;
+    ua.lc_table = lc.p;
+    ua.partials = partials.p;
+
+    int lanes = 1, spl = 1;
+    union_shape(int(r), int(log2m), &lanes, &spl);
+    const uint64_t npb = kBlock / lanes;
+    uint32_t grid = max_grid;
+    if (!list_mode) grid = uint32_t(std::min<uint64_t>(max_grid, std::max<uint64_t>(8, (n + npb - 1) / npb)));
+    grid = (grid + 7) / 8 * 8;
+    if (timing) HB_HIP(hipEventRecord(ev[1], stream));
+    HB_HIP(launch_union(int(r), int(log2m), ua, grid, stream));
+    if (timing) HB_HIP(hipEventRecord(ev[2], stream));
+    HB_HIP(launch_reduce(partials.p, grid, list_mode ? list_len.p : nullptr, stats.p, stream));
+    HB_HIP(hipMemcpyAsync(h_stats, stats.p, sizeof(Stats), hipMemcpyDeviceToHost, stream));
+    if (timing) HB_HIP(hipEventRecord(ev[3], stream));
+    HB_HIP(hipStreamSynchronize(stream));
+    const Stats st = *h_stats;
+
+    hb_iteration_stats hs{};
+    hs.iteration = iteration;
+    hs.systolic = systolic;
+    hs.local = local;
+    hs.pre_local = pre_local;
+    hs.checked = st.checked;
+    hs.visited_arcs = st.visited;
+    hs.scanned_arcs = st.scanned;
+    hs.transpose_arcs = st.tscanned;
+    hs.copied = st.copied;
+    hs.modified = st.modified;
+    if (timing) {
+      float ms = 0;
+      HB_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
+      hs.union_ms = ms;
+      HB_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
+      hs.iteration_ms = ms;
+    }
+
+    cur = 1 - cur;
+    mcur = 1 - mcur;
+    if (systolic) mustcur = 1 - mustcur;
+    double current = base_nf + st.nf;
+    last = current;
+    const double last_output = nf.back();
+    if (current < last_output) current = last_output;
+    rel_inc = current / last_output;
+    nf.push_back(current);
+    hs.nf = current;
+    history.push_back(hs);
+    modified = st.modified;
+    iteration++;
+  }
+
+  void run(uint64_t upper_bound, double threshold) {
+    upper_bound = std::min(upper_bound, n);
+    init();
+    for (uint64_t i = 0; i < upper_bound; i++) {
+      iterate();
+      if (modified == 0) break;
+      if (!std::isnan(threshold) && i > 3 && rel_inc < 1.0 + threshold) break;
+    }
+  }
+
+  void ensure_iteration() const {
+    if (!inited || iteration == 0) fail(HB_E_NOT_RUN, kNotRun);
+  }
+
+  std::vector<double> read_acc(const DevBuf<uint8_t> &buf) {
+    use_device();
+    std::vector<double> out(n);
+    if (n == 0) return out;
+    if (acc_mode == ACC_F64) {
+      HB_HIP(hipMemcpyAsync(out.data(), buf.p, n * 8, hipMemcpyDeviceToHost, stream));
+      HB_HIP(hipStreamSynchronize(stream));
+    } else if (acc_mode == ACC_F32_KAHAN) {
+      std::vector<float> tmp(2 * n);
+      HB_HIP(hipMemcpyAsync(tmp.data(), buf.p, n * 8, hipMemcpyDeviceToHost, stream));
+      HB_HIP(hipStreamSynchronize(stream));
+      for (uint64_t v = 0; v < n; v++) out[v] = double(tmp[2 * v]);
+    } else {
+      std::vector<float> tmp(n);
+      HB_HIP(hipMemcpyAsync(tmp.data(), buf.p, n * 4, hipMemcpyDeviceToHost, stream));
+      HB_HIP(hipStreamSynchronize(stream));
+      for (uint64_t v = 0; v < n; v++) out[v] = double(tmp[v]);
+    }
+    return out;
+  }
+
+  std::vector<double> read_counts() {
+    use_device();
+    std::vector<double> out(n);
+    if (n) {
+      HB_HIP(hipMemcpyAsync(out.data(), cache.p, n * 8, hipMemcpyDeviceToHost, stream));
+      HB_HIP(hipStreamSynchronize(stream));
+    }
+    return out;
+  }
+};
+
+namespace {
+
+void check_csr(const hb_csr *g, const char *what) {
+  if (!g) fail(HB_E_INVALID_ARG, std::string(what) + " is null");
+  if (g->num_nodes > 0xffffffffull)
+    fail(HB_E_INVALID_ARG, std::string(what) + ": more than 2^32 - 1 nodes");
+  if (!g->offsets) fail(HB_E_INVALID_ARG, std::string(what) + ": null offsets");
+  if (g->num_arcs && !g->successors) fail(HB_E_INVALID_ARG, std::string(what) + ": null successors");
+  const uint64_t n = g->num_nodes;
+  if (g->offsets[0] != 0 || g->offsets[n] != g->num_arcs)
+    fail(HB_E_INVALID_ARG, std::string(what) + ": offsets must run from 0 to num_arcs");
+  parallel_for(n, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t v = b; v < e; v++)
+      if (g->offsets[v + 1] < g->offsets[v])
+        fail(HB_E_INVALID_ARG, std::string(what) + ": offsets are not monotone");
+  });
+  parallel_for(g->num_arcs, [&](uint64_t b, uint64_t e, unsigned) {
+    for (uint64_t k = b; k < e; k++)
+      if (g->successors[k] >= n)
+        fail(HB_E_INVALID_ARG, std::string(what) + ": successor out of range");
+  });
+}
+
+std::string min_alignment(uint64_t bits) {
+  if (bits % 128 == 0) return "u128";
+  if (bits % 64 == 0) return "u64";
+  if (bits % 32 == 0) return "u32";
+  if (bits % 16 == 0) return "u16";
+  return "u8";
+}
+
+template <class T>
+void upload(DevBuf<T> &buf, const T *src, size_t count, hipStream_t s) {
+  buf.alloc(count);
+  if (count) HB_HIP(hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+Instance *checked(hb_instance *hb) {
+  if (!hb) fail(HB_E_INVALID_ARG, "null instance");
+  return reinterpret_cast<Instance *>(hb);
+}
+
+void copy_out(const std::vector<double> &v, double *out, size_t cap) {
+  if (!out) fail(HB_E_INVALID_ARG, "null output buffer");
+  if (cap < v.size())
+    fail(HB_E_INVALID_ARG, "output buffer holds " + std::to_string(cap) + " values, " +
+                               std::to_string(v.size()) + " needed");
+  std::memcpy(out, v.data(), v.size() * sizeof(double));
+}
+
+}  // namespace
+}  // namespace hb
+
+using namespace hb;
+
+extern "C" {
+
+uint32_t hb_register_size(uint64_t n) { return hb::register_size(n); }
+double hb_rel_std(uint32_t log2m) { return hb::rel_std(log2m); }
+uint32_t hb_log2_num_registers(double rsd) { return hb::log2_num_registers(rsd); }
+
+void hb_config_init(hb_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->log2m = 6;
+  cfg->hash = HB_HASH_SIPHASH13;
+  cfg->arc_granularity = 16 * 1024;
+  cfg->acc_precision = HB_ACC_F64;
+}
+
+hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_config *cfg,
+                    hb_instance **out) {
+  return guard([&] {
+    if (!out) fail(HB_E_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    if (!cfg) fail(HB_E_INVALID_ARG, "null configuration");
+    check_csr(graph, "graph");
+    const uint64_t n = graph->num_nodes;
+    if (transpose) {
+      check_csr(transpose, "transpose");
+      if (transpose->num_nodes != n)
+        fail(HB_E_INVALID_ARG, "the transpose should have same number of nodes of the graph (" +
+                                   std::to_string(n) + "). Got " +
+                                   std::to_string(transpose->num_nodes) + ".");
+      if (transpose->num_arcs != graph->num_arcs)
+        fail(HB_E_INVALID_ARG, "the transpose should have same number of nodes of the graph (" +
+                                   std::to_string(graph->num_arcs) + "). Got " +
+                                   std::to_string(transpose->num_arcs) + ".");
+    }
+    if (cfg->acc_precision > HB_ACC_F32) fail(HB_E_INVALID_ARG, "unknown accumulator precision");
+    if (cfg->hash > HB_HASH_JENKINS) fail(HB_E_INVALID_ARG, "unknown hash");
+    uint64_t num_elements = cfg->num_elements ? cfg->num_elements : n;
+    if (cfg->weights && !cfg->num_elements) {
+      num_elements = 0;
+      for (uint64_t v = 0; v < n; v++) num_elements += cfg->weights[v];
+    }
+    if (cfg->log2m < 4)
+      fail(HB_E_INVALID_ARG,
+           "the logarithm of the number of registers per counter should be at least 4; got " +
+               std::to_string(cfg->log2m));
+    const uint32_t r = cfg->reg_bits ? cfg->reg_bits : register_size(num_elements);
+    if (cfg->log2m > 30) fail(HB_E_INVALID_ARG, "log2m out of range");
+    const uint64_t m = uint64_t(1) << cfg->log2m;
+    if ((m * r) % 64 != 0)
+      fail(HB_E_INVALID_ARG,
+           "W should allow counter backends to be aligned. Use " + min_alignment(m * r) +
+               " or smaller unsigned integer types");
+    int lanes, spl;
+    if (!union_shape(int(r), int(cfg->log2m), &lanes, &spl))
+      fail(HB_E_INVALID_ARG, "no kernel for register width " + std::to_string(r) + " and log2m " +
+                                 std::to_string(cfg->log2m) +
+                                 " (this build holds widths 5 and 6, log2m 5..12)");
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      fail(HB_E_NO_DEVICE, "no HIP device is visible");
+    }
+    if (cfg->device < 0 || cfg->device >= ndev)
+      fail(HB_E_NO_DEVICE, "device " + std::to_string(cfg->device) + " does not exist");
+    hipDeviceProp_t prop;
+    HB_HIP(hipGetDeviceProperties(&prop, cfg->device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      fail(HB_E_NO_DEVICE, std::string("device is ") + prop.gcnArchName +
+                               "; this library is built for gfx950 only");
+
+    auto inst = std::make_unique<Instance>();
+    Instance &I = *inst;
+    I.device = cfg->device;
+    I.use_device();
+    I.cus = uint32_t(prop.multiProcessorCount);
+    HB_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+    for (auto &e : I.ev) HB_HIP(hipEventCreate(&e));
+    void *hs = nullptr;
+    HB_HIP(hipHostMalloc(&hs, sizeof(Stats), hipHostMallocDefault));
+    I.h_stats = static_cast<Stats *>(hs);
+
+    I.n = n;
+    I.a = graph->num_arcs;
+    I.log2m = cfg->log2m;
+    I.m = uint32_t(m);
+    I.r = r;
+    I.W = uint32_t(m * r / 64);
+    I.cdw = uint32_t(m * r / 32);
+    I.sentinel = uint64_t(1) << ((1u << r) - 2);
+    I.amm = alpha_m_m(cfg->log2m);
+    I.hash = cfg->hash;
+    I.seed = cfg->seed;
+    I.weight_seed = cfg->weight_seed;
+    I.weighted = cfg->weights != nullptr;
+    I.has_t = transpose != nullptr;
+    I.do_sd = cfg->sum_of_distances != 0;
+    I.do_hc = cfg->sum_of_inverse_distances != 0;
+    I.acc_mode = cfg->acc_precision;
+
+    hipStream_t s = I.stream;
+    upload(I.off, graph->offsets, n + 1, s);
+    upload(I.succ, graph->successors, graph->num_arcs, s);
+    if (transpose) {
+      upload(I.toff, transpose->offsets, n + 1, s);
+      upload(I.tsucc, transpose->successors, transpose->num_arcs, s);
+    }
+    std::vector<uint64_t> prefix;
+    if (I.weighted) {
+      prefix.resize(n + 1, 0);
+      for (uint64_t v = 0; v < n; v++) prefix[v + 1] = prefix[v] + cfg->weights[v];
+      upload(I.d_weights, cfg->weights, n, s);
+      upload(I.d_wprefix, prefix.data(), n + 1, s);
+    }
+    std::vector<double> lc(m + 1, 0.0);
+    for (uint64_t z = 1; z <= m; z++) lc[z] = double(m) * std::log(double(m) / double(z));
+    upload(I.lc, lc.data(), m + 1, s);
+
+    I.bitmap_words = std::max<uint64_t>(2, ((n + 63) / 64) * 2);
+    for (int k = 0; k < 2; k++) {
+      I.cnt[k].alloc(size_t(n) * I.cdw);
+      I.mod[k].alloc(I.bitmap_words);
+      I.must[k].alloc(I.bitmap_words);
+    }
+    I.cache.alloc(n);
+    if (I.do_sd) I.sd.alloc(n * I.acc_bytes());
+    if (I.do_hc) I.hc.alloc(n * I.acc_bytes());
+    I.list.alloc(n);
+    I.bsum.alloc(compact_scratch_words(n));
+    I.list_len.alloc(1);
+    I.max_grid = std::max<uint32_t>(64, I.cus * 8);
+    I.max_grid = (I.max_grid + 7) / 8 * 8;
+    I.partials.alloc(I.max_grid);
+    I.stats.alloc(1);
+    HB_HIP(hipStreamSynchronize(s));
+    *out = reinterpret_cast<hb_instance *>(inst.release());
+  });
+}
+
+hb_status hb_destroy(hb_instance *hb) {
+  return guard([&] { delete reinterpret_cast<Instance *>(hb); });
+}
+
+hb_status hb_add_discount_function(hb_instance *hb, hb_discount_fn f, void *ctx) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    if (!f) fail(HB_E_INVALID_ARG, "null discount function");
+    if (I.discounts.size() >= size_t(kMaxDiscount))
+      fail(HB_E_INVALID_ARG, "at most " + std::to_string(kMaxDiscount) + " discount functions");
+    I.use_device();
+    auto buf = std::make_unique<DevBuf<uint8_t>>();
+    buf->alloc(I.n * I.acc_bytes());
+    I.fill_zero(buf->p, I.n * I.acc_bytes());
+    HB_HIP(hipStreamSynchronize(I.stream));
+    I.dc.push_back(std::move(buf));
+    I.discounts.push_back({f, ctx});
+  });
+}
+
+hb_status hb_clear_discount_functions(hb_instance *hb) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.use_device();
+    HB_HIP(hipStreamSynchronize(I.stream));
+    I.discounts.clear();
+    I.dc.clear();
+  });
+}
+
+hb_status hb_run(hb_instance *hb, uint64_t upper_bound, double threshold) {
+  return guard([&] { checked(hb)->run(upper_bound, threshold); });
+}
+
+hb_status hb_run_until_stable(hb_instance *hb, uint64_t upper_bound) {
+  return guard([&] { checked(hb)->run(upper_bound, NAN); });
+}
+
+hb_status hb_run_until_done(hb_instance *hb) {
+  return guard([&] { checked(hb)->run(UINT64_MAX, NAN); });
+}
+
+hb_status hb_init(hb_instance *hb) {
+  return guard([&] { checked(hb)->init(); });
+}
+
+hb_status hb_iterate(hb_instance *hb) {
+  return guard([&] { checked(hb)->iterate(); });
+}
+
+hb_status hb_neighborhood_function(hb_instance *hb, double *out, size_t cap, size_t *len) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.ensure_iteration();
+    if (len) *len = I.nf.size();
+    if (out) copy_out(I.nf, out, cap);
+  });
+}
+
+hb_status hb_sum_of_distances(hb_instance *hb, double *out, size_t n) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.ensure_iteration();
+    if (!I.do_sd)
+      fail(HB_E_NOT_REQUESTED,
+           "Sum of distances were not requested. Use builder.with_sum_of_distances(true) while "
+           "building HyperBall to compute them");
+    copy_out(I.read_acc(I.sd), out, n);
+  });
+}
+
+hb_status hb_harmonic_centralities(hb_instance *hb, double *out, size_t n) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.ensure_iteration();
+    if (!I.do_hc)
+      fail(HB_E_NOT_REQUESTED,
+           "Sum of inverse distances were not requested. Use "
+           "builder.with_sum_of_inverse_distances(true) while building HyperBall to compute them");
+    copy_out(I.read_acc(I.hc), out, n);
+  });
+}
+
+hb_status hb_discounted_centrality(hb_instance *hb, size_t index, double *out, size_t n) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.ensure_iteration();
+    if (index >= I.dc.size())
+      fail(HB_E_NOT_REQUESTED, "Discount centrality of index " + std::to_string(index) +
+                                   " does not exist");
+    copy_out(I.read_acc(*I.dc[index]), out, n);
+  });
+}
+
+static void derived(hb_instance *hb, int which, double *out, size_t cap) {
+  Instance &I = *checked(hb);
+  I.ensure_iteration();
+  if (!I.do_sd) {
+    static const char *what[3] = {"closeness centrality", "lin centrality", "lin centrality"};
+    fail(HB_E_NOT_REQUESTED,
+         std::string("Sum of distances were not requested. Use builder.with_sum_of_distances(true) "
+                     "while building HyperBall to compute ") + what[which]);
+  }
+  std::vector<double> d = I.read_acc(I.sd);
+  std::vector<double> c;
+  if (which != 0) c = I.read_counts();
+  for (uint64_t v = 0; v < I.n; v++) {
+    if (which == 0) d[v] = d[v] == 0.0 ? 0.0 : 1.0 / d[v];
+    else if (which == 1) d[v] = d[v] == 0.0 ? 1.0 : c[v] * c[v] / d[v];
+    else d[v] = c[v] * c[v] - d[v];
+  }
+  copy_out(d, out, cap);
+}
+
+hb_status hb_closeness_centrality(hb_instance *hb, double *out, size_t n) {
+  return guard([&] { derived(hb, 0, out, n); });
+}
+hb_status hb_lin_centrality(hb_instance *hb, double *out, size_t n) {
+  return guard([&] { derived(hb, 1, out, n); });
+}
+hb_status hb_nieminen_centrality(hb_instance *hb, double *out, size_t n) {
+  return guard([&] { derived(hb, 2, out, n); });
+}
+
+hb_status hb_reachable_nodes(hb_instance *hb, double *out, size_t n) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.ensure_iteration();
+    copy_out(I.read_counts(), out, n);
+  });
+}
+
+hb_status hb_reachable_nodes_from(hb_instance *hb, uint64_t node, double *out) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.ensure_iteration();
+    if (!out) fail(HB_E_INVALID_ARG, "null output");
+    if (node >= I.n) fail(HB_E_INVALID_ARG, "node " + std::to_string(node) + " out of range");
+    I.use_device();
+    HB_HIP(hipMemcpyAsync(out, I.cache.p + node, 8, hipMemcpyDeviceToHost, I.stream));
+    HB_HIP(hipStreamSynchronize(I.stream));
+  });
+}
+
+hb_status hb_get_params(hb_instance *hb, hb_params *out) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    if (!out) fail(HB_E_INVALID_ARG, "null output");
+    out->log2m = I.log2m;
+    out->num_registers = I.m;
+    out->register_bits = I.r;
+    out->words_per_counter = I.W;
+    out->num_nodes = I.n;
+    out->num_arcs = I.a;
+    out->alpha_m_m = I.amm;
+  });
+}
+
+hb_status hb_get_counters(hb_instance *hb, int which, uint64_t *out, size_t words) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    if (!out) fail(HB_E_INVALID_ARG, "null output");
+    if (!I.inited) fail(HB_E_NOT_RUN, "counters are not initialised; call hb_init or hb_run");
+    if (words < I.n * I.W) fail(HB_E_INVALID_ARG, "output buffer too small");
+    I.use_device();
+    const uint32_t *src = I.cnt[which ? 1 - I.cur : I.cur].p;
+    const uint64_t chunk = std::max<uint64_t>(1, (uint64_t(64) << 20) / (I.W * 8));
+    DevBuf<uint64_t> tmp;
+    tmp.alloc(std::min<uint64_t>(I.n, chunk) * I.W);
+    for (uint64_t b = 0; b < I.n; b += chunk) {
+      const uint64_t c = std::min(chunk, I.n - b);
+      HB_HIP(launch_to_reference_layout(src, b, c, I.r, I.m, tmp.p, I.stream));
+      HB_HIP(hipMemcpyAsync(out + b * I.W, tmp.p, c * I.W * 8, hipMemcpyDeviceToHost, I.stream));
+      HB_HIP(hipStreamSynchronize(I.stream));
+    }
+  });
+}
+
+hb_status hb_get_modified(hb_instance *hb, uint64_t *out, size_t words) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    if (!out) fail(HB_E_INVALID_ARG, "null output");
+    if (!I.inited) fail(HB_E_NOT_RUN, "not initialised; call hb_init or hb_run");
+    const uint64_t need = (I.n + 63) / 64;
+    if (words < need) fail(HB_E_INVALID_ARG, "output buffer too small");
+    I.use_device();
+    if (need) {
+      HB_HIP(hipMemcpyAsync(out, I.mod[I.mcur].p, need * 8, hipMemcpyDeviceToHost, I.stream));
+      HB_HIP(hipStreamSynchronize(I.stream));
+    }
+  });
+}
+
+hb_status hb_num_iterations(hb_instance *hb, uint64_t *out) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    if (!out) fail(HB_E_INVALID_ARG, "null output");
+    *out = I.history.size();
+  });
+}
+
+hb_status hb_get_iteration_stats(hb_instance *hb, uint64_t iteration, hb_iteration_stats *out) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    if (!out) fail(HB_E_INVALID_ARG, "null output");
+    if (iteration >= I.history.size()) fail(HB_E_INVALID_ARG, "no such iteration");
+    *out = I.history[iteration];
+  });
+}
+
+hb_status hb_set_timing(hb_instance *hb, int enabled) {
+  return guard([&] { checked(hb)->timing = enabled != 0; });
+}
+
+hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree) {
+  return guard([&] { checked(hb)->heavy_degree = heavy_degree; });
+}
+
+}  // extern "C"

@@ -1,0 +1,681 @@
+// HyperBall kernels for gfx950 (MI355X, CDNA4, wave64).
+//
+// Hot path: union_kernel, one iteration of HyperBall over the nodes it is
+// given (all nodes, or a compacted worklist).  It restates the per-node body
+// of the reference's parallel_task (src/algo/hyperball/hyperball_impl.rs:
+// 1025-1145) on a lane group per node:
+//   - the group reads G successor ids at a time, tests their modified bits,
+//     and a group ballot yields the modified successors (hyperball_impl.rs:
+//     1037-1049);
+//   - each lane owns SPL slabs of the node's counter and merges the same
+//     slabs of every modified successor (per-register maximum, the
+//     definition of merge_hyperloglog_bitwise, hyper_log_log_logic.rs:
+//     469-576), up to four successor loads in flight;
+//   - the changed test, count(), the neighbourhood-function contribution,
+//     centrality updates, modified bit and predecessor marking follow
+//     :1052-1130; the new counter is stored when it changed or when the other
+//     buffer holds a stale copy (:1132-1144).
+// count() results are cached per node (count of the current counter), so a
+// counter is counted once per change, and the pre-change count (:1066) is a
+// read of the cache.
+#include "hb_kernels.hpp"
+
+#include "hb_hll.hpp"
+
+namespace hb {
+namespace {
+
+__device__ __forceinline__ bool test_bit(const uint32_t *bm, uint32_t v) {
+  return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+template <int G>
+__device__ __forceinline__ uint64_t group_ballot(bool pred) {
+  uint64_t b = __ballot(pred);
+  if constexpr (G == 64) {
+    return b;
+  } else {
+    const int base = __lane_id() & ~(G - 1);
+    return (b >> base) & ((uint64_t(1) << G) - 1);
+  }
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t group_shfl(uint32_t v, int src) {
+  if constexpr (G == 1) return v;
+  else return __shfl(v, src, G);
+}
+
+template <int G>
+__device__ __forceinline__ uint64_t group_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+// Per-register maximum of one slab: x = max(x, y) over 32 registers held as
+// R bit planes.  gt marks the registers where x > y, decided from the top
+// plane down.
+template <int R>
+__device__ __forceinline__ void slab_max(uint32_t *x, const uint32_t *y) {
+  uint32_t gt = x[R - 1] & ~y[R - 1];
+  uint32_t eq = ~(x[R - 1] ^ y[R - 1]);
+#pragma unroll
+  for (int b = R - 2; b >= 1; b--) {
+    gt |= eq & x[b] & ~y[b];
+    eq &= ~(x[b] ^ y[b]);
+  }
+  gt |= eq & x[0] & ~y[0];
+#pragma unroll
+  for (int b = 0; b < R; b++) x[b] = (x[b] & gt) | (y[b] & ~gt);
+}
+
+// count() pieces of one slab.
+//   r = 5: S = sum 2^(31 - v) as a signed integer, with the reference's
+//          -1 for v = 31 (hyper_log_log_logic.rs:178 shifts an i32);
+//   r = 6: S = sum 2^(63 - v) in 128 bits (exact 2^-v).
+// Registers are grouped by v >> 2 (class h) and v & 3; class masks come from
+// the upper planes, so the sum is a few popcounts per class.
+struct SlabSum5 {
+  int64_t s;
+  uint32_t zeros;
+};
+
+__device__ __forceinline__ SlabSum5 slab_sum5(const uint32_t *p) {
+  SlabSum5 out;
+  out.zeros = __popc(~(p[0] | p[1] | p[2] | p[3] | p[4]));
+  const uint32_t n2 = ~p[2], n3 = ~p[3], n4 = ~p[4];
+  const uint32_t q[4] = {n4 & n3, n4 & p[3], p[4] & n3, p[4] & p[3]};
+  uint64_t s = 0;
+  uint32_t quirk = 0;
+#pragma unroll
+  for (int h = 0; h < 8; h++) {
+    const uint32_t mh = q[h >> 1] & ((h & 1) ? p[2] : n2);
+    const uint32_t a = mh & ~p[1], b = mh & p[1];
+    const uint32_t pa = __popc(a), pa0 = __popc(a & p[0]), pb = __popc(b), pb0 = __popc(b & p[0]);
+    const uint64_t t = uint64_t(8 * pa - 4 * pa0 + 2 * pb - pb0);
+    s += t << (28 - 4 * h);
+    if (h == 7) quirk = pb0;
+  }
+  out.s = int64_t(s) - 2 * int64_t(quirk);
+  return out;
+}
+
+struct SlabSum6 {
+  unsigned __int128 s;
+  uint32_t zeros;
+};
+
+__device__ __forceinline__ SlabSum6 slab_sum6(const uint32_t *p) {
+  SlabSum6 out;
+  out.zeros = __popc(~(p[0] | p[1] | p[2] | p[3] | p[4] | p[5]));
+  const uint32_t n[6] = {~p[0], ~p[1], ~p[2], ~p[3], ~p[4], ~p[5]};
+  uint32_t q[16];
+#pragma unroll
+  for (int h = 0; h < 16; h++) {
+    q[h] = ((h & 8) ? p[5] : n[5]) & ((h & 4) ? p[4] : n[4]) & ((h & 2) ? p[3] : n[3]) &
+           ((h & 1) ? p[2] : n[2]);
+  }
+  unsigned __int128 s = 0;
+#pragma unroll
+  for (int h = 0; h < 16; h++) {
+    const uint32_t a = q[h] & n[1], b = q[h] & p[1];
+    const uint32_t t = 8 * __popc(a) - 4 * __popc(a & p[0]) + 2 * __popc(b) - __popc(b & p[0]);
+    s += (unsigned __int128)t << (60 - 4 * h);
+  }
+  out.s = s;
+  return out;
+}
+
+// count() of the counter whose slabs are spread over the G lanes of a group;
+// every lane returns the result.  hyper_log_log_logic.rs:169-186.
+template <int R, int G, int SPL>
+__device__ __forceinline__ double group_count(const uint32_t *x, double alpha_m_m, uint32_t m,
+                                              const double *lc_table) {
+  double hm;
+  uint32_t zeros = 0;
+  if constexpr (R == 5) {
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SPL; k++) {
+      SlabSum5 t = slab_sum5(x + k * R);
+      s += t.s;
+      zeros += t.zeros;
+    }
+    s = int64_t(group_sum_u64<G>(uint64_t(s)));
+    zeros = group_sum_u32<G>(zeros);
+    hm = double(s) * 0x1p-31;
+  } else {
+    unsigned __int128 s = 0;
+#pragma unroll
+    for (int k = 0; k < SPL; k++) {
+      SlabSum6 t = slab_sum6(x + k * R);
+      s += t.s;
+      zeros += t.zeros;
+    }
+    uint64_t lo = uint64_t(s), hi = uint64_t(s >> 64);
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+      const uint64_t olo = __shfl_xor(lo, o, G), ohi = __shfl_xor(hi, o, G);
+      const uint64_t nlo = lo + olo;
+      hi = hi + ohi + (nlo < lo ? 1 : 0);
+      lo = nlo;
+    }
+    zeros = group_sum_u32<G>(zeros);
+    hm = (double(hi) * 18446744073709551616.0 + double(lo)) * 0x1p-63;
+  }
+  double est = alpha_m_m / hm;
+  if (zeros != 0 && est < 2.5 * double(m)) est = lc_table[zeros];
+  return est;
+}
+
+template <int D>
+__device__ __forceinline__ void load_part(uint32_t *dst, const uint32_t *src) {
+#pragma unroll
+  for (int k = 0; k < D; k++) dst[k] = src[k];
+}
+
+template <int D>
+__device__ __forceinline__ void store_part(uint32_t *dst, const uint32_t *src) {
+#pragma unroll
+  for (int k = 0; k < D; k++) dst[k] = src[k];
+}
+
+struct Acc {
+  double nf = 0.0;
+  uint64_t visited = 0, scanned = 0, tscanned = 0;
+  uint32_t checked = 0, copied = 0, modified = 0;
+};
+
+__device__ __forceinline__ void kahan_add(float2 *p, float x) {
+  float2 v = *p;
+  const float y = x - v.y;
+  const float t = v.x + y;
+  v.y = (t - v.x) - y;
+  v.x = t;
+  *p = v;
+}
+
+__device__ __forceinline__ void acc_add(void *base, uint32_t mode, uint32_t v, double x) {
+  if (mode == ACC_F64) {
+    double *p = static_cast<double *>(base) + v;
+    *p = *p + x;
+  } else if (mode == ACC_F32_KAHAN) {
+    kahan_add(static_cast<float2 *>(base) + v, float(x));
+  } else {
+    float *p = static_cast<float *>(base) + v;
+    *p = *p + float(x);
+  }
+}
+
+// Block-wide sum of the per-thread accumulators; the result is valid in
+// thread 0.  Every thread of the block must call it.
+__device__ Partial block_sum(const Acc &a) {
+  __shared__ double s_nf[kBlock / 64];
+  __shared__ uint64_t s_u[3][kBlock / 64];
+  __shared__ uint32_t s_c[3][kBlock / 64];
+  double nf = a.nf;
+  uint64_t u0 = a.visited, u1 = a.scanned, u2 = a.tscanned;
+  uint32_t c0 = a.checked, c1 = a.copied, c2 = a.modified;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    nf += __shfl_xor(nf, o);
+    u0 += __shfl_xor(u0, o);
+    u1 += __shfl_xor(u1, o);
+    u2 += __shfl_xor(u2, o);
+    c0 += __shfl_xor(c0, o);
+    c1 += __shfl_xor(c1, o);
+    c2 += __shfl_xor(c2, o);
+  }
+  const int w = threadIdx.x / 64;
+  if (__lane_id() == 0) {
+    s_nf[w] = nf;
+    s_u[0][w] = u0; s_u[1][w] = u1; s_u[2][w] = u2;
+    s_c[0][w] = c0; s_c[1][w] = c1; s_c[2][w] = c2;
+  }
+  __syncthreads();
+  Partial p{};
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < kBlock / 64; k++) {
+      p.nf += s_nf[k];
+      p.visited += s_u[0][k]; p.scanned += s_u[1][k]; p.tscanned += s_u[2][k];
+      p.checked += s_c[0][k]; p.copied += s_c[1][k]; p.modified += s_c[2][k];
+    }
+  }
+  return p;
+}
+
+// Physical workgroup ids are dealt round-robin to the 8 XCDs; remap them so
+// that each XCD walks a contiguous range of node batches in every round and
+// window-local successor reads hit that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_logical_block() {
+  const uint32_t nb = gridDim.x, pb = blockIdx.x;
+  return (pb % 8) * (nb / 8) + pb / 8;
+}
+
+template <int R, int G, int SPL>
+__device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int gl, Acc &acc_out) {
+  constexpr int D = R * SPL;         // dwords per lane
+  constexpr int CD = G * D;          // dwords per counter
+  constexpr int U = D <= 12 ? 4 : 2; // successor loads in flight per lane
+  const bool systolic = a.flags & F_SYSTOLIC;
+  const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
+  const bool was_mod = test_bit(a.curr_mod, v);
+  if (!check && !was_mod) return;
+
+  uint32_t prev[D], cur[D];
+  bool loaded = false;
+  uint32_t nvisit = 0;
+  uint64_t deg = 0;
+  const uint32_t *own = a.curr + uint64_t(v) * CD + gl * D;
+
+  if (check) {
+    const uint64_t beg = a.off[v], end = a.off[v + 1];
+    deg = end - beg;
+    for (uint64_t base = beg; base < end; base += G) {
+      const uint64_t idx = base + gl;
+      const uint32_t s = idx < end ? a.succ[idx] : v;
+      const bool f = (s != v) && test_bit(a.curr_mod, s);
+      uint64_t mask = group_ballot<G>(f);
+      if (!mask) continue;
+      if (!loaded) {
+        load_part<D>(prev, own);
+#pragma unroll
+        for (int k = 0; k < D; k++) cur[k] = prev[k];
+        loaded = true;
+      }
+      nvisit += __popcll(mask);
+      while (mask) {
+        uint32_t t[U];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          if (mask) {
+            const int b = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            t[u] = group_shfl<G>(s, b);
+            cnt++;
+          } else {
+            t[u] = 0;
+          }
+        }
+        uint32_t y[U][D];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (u < cnt) load_part<D>(y[u], a.curr + uint64_t(t[u]) * CD + gl * D);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (u < cnt) {
+#pragma unroll
+            for (int k = 0; k < SPL; k++) slab_max<R>(cur + k * R, y[u] + k * R);
+          }
+      }
+    }
+  }
+
+  bool cm = false;
+  if (loaded) {
+    uint32_t d = 0;
+#pragma unroll
+    for (int k = 0; k < D; k++) d |= cur[k] ^ prev[k];
+    cm = group_ballot<G>(d != 0) != 0;
+  }
+  double post = 0.0;
+  if (cm) post = group_count<R, G, SPL>(cur, a.alpha_m_m, a.m, a.lc_table);
+
+  if (gl == 0) {
+    double pre = 0.0;
+    if (cm) {
+      pre = a.cache[v];
+      a.cache[v] = post;
+    } else if (!systolic && check) {
+      post = a.cache[v];
+    }
+    if (!systolic && check) acc_out.nf += post;
+    if (cm) {
+      if (systolic) acc_out.nf += post - pre;
+      if (a.flags & F_CENTRALITY) {
+        const double delta = post - pre;
+        if (delta > 0.0) {
+          if (a.sd) acc_add(a.sd, a.acc_mode, v, delta * a.dist);
+          if (a.hc) acc_add(a.hc, a.acc_mode, v, delta / a.dist);
+          for (int k = 0; k < a.ndisc; k++) acc_add(a.dc[k], a.acc_mode, v, delta * a.disc[k]);
+        }
+      }
+      atomicOr(a.next_mod + (v >> 5), 1u << (v & 31));
+      acc_out.modified++;
+    }
+    acc_out.visited += nvisit;
+    acc_out.scanned += deg;
+    if (check) acc_out.checked++;
+    else acc_out.copied++;
+  }
+
+  if (cm && systolic) {
+    const uint64_t tb = a.toff[v], te = a.toff[v + 1];
+    for (uint64_t k = tb + gl; k < te; k += G) {
+      const uint32_t p = a.tsucc[k];
+      atomicOr(a.next_must + (p >> 5), 1u << (p & 31));
+    }
+    if (gl == 0) acc_out.tscanned += te - tb;
+  }
+
+  if (cm || was_mod) {
+    if (!loaded) load_part<D>(cur, own);
+    store_part<D>(a.next + uint64_t(v) * CD + gl * D, cur);
+  }
+}
+
+template <int R, int G, int SPL>
+__global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
+  constexpr int NPB = kBlock / G;  // nodes per block and batch
+  const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
+  Acc acc;
+  for (uint64_t batch = xcd_logical_block(); batch * NPB < count; batch += gridDim.x) {
+    const uint64_t i = batch * NPB + g;
+    if (i < count) {
+      const uint32_t v = (a.flags & F_LIST) ? a.list[i] : uint32_t(i);
+      process_node<R, G, SPL>(a, v, gl, acc);
+    }
+  }
+  const Partial p = block_sum(acc);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = p;
+}
+
+template <int R, int G, int SPL>
+__global__ __launch_bounds__(kBlock) void count_all_kernel(const uint32_t *cnt, uint64_t n,
+                                                          double alpha_m_m, uint32_t m,
+                                                          const double *lc, double *out) {
+  constexpr int D = R * SPL, CD = G * D, NPB = kBlock / G;
+  const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  for (uint64_t batch = blockIdx.x; batch * NPB < n; batch += gridDim.x) {
+    const uint64_t i = batch * NPB + g;
+    if (i < n) {
+      uint32_t x[D];
+      load_part<D>(x, cnt + i * CD + gl * D);
+      const double c = group_count<R, G, SPL>(x, alpha_m_m, m, lc);
+      if (gl == 0) out[i] = c;
+    }
+  }
+}
+
+// HyperBall::init seeding (hyperball_impl.rs:1167-1178) with
+// HyperLogLog::add (hyper_log_log_logic.rs:151-167), one thread per node.
+__global__ __launch_bounds__(kBlock) void seed_kernel(SeedArgs a) {
+  for (uint64_t v = uint64_t(blockIdx.x) * kBlock + threadIdx.x; v < a.n;
+       v += uint64_t(gridDim.x) * kBlock) {
+    uint32_t *c = a.counters + v * a.cdw;
+    const uint64_t items = a.weights ? a.weights[v] : 1;
+    const uint64_t first = a.weights ? a.weight_prefix[v] : 0;
+    for (uint64_t k = 0; k < items; k++) {
+      const uint64_t item = a.weights ? weight_item(a.weight_seed, first + k) : v;
+      const uint64_t x = a.hash == 1 ? jenkins(item, a.seed) : siphash13(item);
+      const uint32_t j = uint32_t(x & ((uint64_t(1) << a.log2m) - 1));
+      const uint32_t val = uint32_t(__builtin_ctzll((x >> a.log2m) | a.sentinel)) + 1;
+      uint32_t *slab = c + (j >> 5) * a.r;
+      const uint32_t q = j & 31;
+      uint32_t cur = 0;
+      for (uint32_t b = 0; b < a.r; b++) cur |= ((slab[b] >> q) & 1u) << b;
+      if (val > cur)
+        for (uint32_t b = 0; b < a.r; b++)
+          slab[b] = (slab[b] & ~(1u << q)) | (((val >> b) & 1u) << q);
+    }
+  }
+}
+
+// Deterministic sum of the block partials.
+__global__ __launch_bounds__(kBlock) void reduce_kernel(const Partial *p, uint32_t np,
+                                                        const uint32_t *list_len, Stats *out) {
+  Acc a;
+  for (uint32_t i = threadIdx.x; i < np; i += kBlock) {
+    const Partial q = p[i];
+    a.nf += q.nf;
+    a.visited += q.visited; a.scanned += q.scanned; a.tscanned += q.tscanned;
+    a.checked += q.checked; a.copied += q.copied; a.modified += q.modified;
+  }
+  const Partial t = block_sum(a);
+  if (threadIdx.x == 0) {
+    Stats s;
+    s.nf = t.nf;
+    s.visited = t.visited; s.scanned = t.scanned; s.tscanned = t.tscanned;
+    s.checked = t.checked; s.copied = t.copied; s.modified = t.modified;
+    s.list_len = list_len ? *list_len : 0;
+    *out = s;
+  }
+}
+
+// ---- bitmap compaction ---------------------------------------------------
+
+__device__ __forceinline__ uint32_t union_word(const uint32_t *a, const uint32_t *b, uint64_t w,
+                                               uint64_t n) {
+  const uint64_t nw = (n + 31) / 32;
+  if (w >= nw) return 0;
+  uint32_t x = a[w] | (b ? b[w] : 0u);
+  if (w == nw - 1 && (n & 31)) x &= (1u << (n & 31)) - 1;
+  return x;
+}
+
+// Exclusive block scan of one value per thread; returns the block total too.
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total) {
+  __shared__ uint32_t s_w[kBlock / 64];
+  const int lane = __lane_id(), w = threadIdx.x / 64;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  uint32_t before = 0, tot = 0;
+  for (int k = 0; k < kBlock / 64; k++) {
+    if (k < w) before += s_w[k];
+    tot += s_w[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void compact_count_kernel(const uint32_t *a, const uint32_t *b,
+                                                               uint64_t n, uint32_t *bsum) {
+  const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint32_t tot;
+  block_exclusive_scan(__popc(union_word(a, b, w, n)), &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void compact_scan_kernel(uint32_t *bsum, uint32_t nb,
+                                                            uint32_t *len) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  const int lane = __lane_id(), w = threadIdx.x / 64;
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nb ? bsum[i] : 0;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(inc, o);
+      if (lane >= o) inc += t;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = s_carry, tot = 0;
+    for (int k = 0; k < 16; k++) {
+      if (k < w) before += s_w[k];
+      tot += s_w[k];
+    }
+    if (i < nb) bsum[i] = before + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *len = s_carry;
+}
+
+__global__ __launch_bounds__(kBlock) void compact_emit_kernel(const uint32_t *a, const uint32_t *b,
+                                                              uint64_t n, const uint32_t *boff,
+                                                              uint32_t *list) {
+  const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint32_t x = union_word(a, b, w, n);
+  uint32_t tot;
+  uint32_t pos = boff[blockIdx.x] + block_exclusive_scan(__popc(x), &tot);
+  while (x) {
+    const int bit = __builtin_ctz(x);
+    x &= x - 1;
+    list[pos++] = uint32_t(w * 32 + bit);
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t words, uint32_t value) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = value;
+}
+
+__global__ void fill_bits_kernel(uint32_t *p, uint64_t n) {
+  const uint64_t nw = (n + 31) / 32;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nw;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = (i == nw - 1 && (n & 31)) ? (1u << (n & 31)) - 1 : ~0u;
+}
+
+// Slab layout -> reference layout, one thread per output word.
+__global__ void to_reference_kernel(const uint32_t *cnt, uint64_t node_begin, uint64_t node_count,
+                                    uint32_t r, uint32_t m, uint64_t *out) {
+  const uint32_t W = m * r / 64, cdw = m * r / 32;
+  const uint64_t total = node_count * W;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t v = node_begin + i / W;
+    const uint32_t k = uint32_t(i % W);
+    const uint32_t *c = cnt + v * cdw;
+    uint64_t word = 0;
+    for (uint32_t p = 0; p < 64; p++) {
+      const uint32_t g = 64 * k + p, j = g / r, b = g % r;
+      word |= uint64_t((c[(j >> 5) * r + b] >> (j & 31)) & 1u) << p;
+    }
+    out[i] = word;
+  }
+}
+
+uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
+  uint64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return uint32_t(g);
+}
+
+template <int R, int G, int SPL>
+hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, hipStream_t s) {
+  grid = (grid + 7) / 8 * 8;  // xcd_logical_block needs a multiple of 8
+  union_kernel<R, G, SPL><<<grid, kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <int R, int G, int SPL>
+hipError_t launch_count_t(const uint32_t *c, uint64_t n, double amm, uint32_t m, const double *lc,
+                          double *out, hipStream_t s) {
+  count_all_kernel<R, G, SPL><<<grid_for(n, kBlock / G, 65536), kBlock, 0, s>>>(c, n, amm, m, lc, out);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Lane group per (r, log2m): G lanes per node, SPL slabs per lane.
+bool union_shape(int r, int log2m, int *lanes, int *spl) {
+  if ((r != 5 && r != 6) || log2m < 5 || log2m > 12) return false;
+  static const int G[8] = {1, 2, 4, 8, 8, 16, 32, 64};
+  static const int S[8] = {1, 1, 1, 1, 2, 2, 2, 2};
+  *lanes = G[log2m - 5];
+  *spl = S[log2m - 5];
+  return true;
+}
+
+#define HB_SHAPES(X)                                                                    \
+  X(5, 5, 1, 1) X(5, 6, 2, 1) X(5, 7, 4, 1) X(5, 8, 8, 1) X(5, 9, 8, 2) X(5, 10, 16, 2) \
+  X(5, 11, 32, 2) X(5, 12, 64, 2) X(6, 5, 1, 1) X(6, 6, 2, 1) X(6, 7, 4, 1) X(6, 8, 8, 1)  \
+  X(6, 9, 8, 2) X(6, 10, 16, 2) X(6, 11, 32, 2) X(6, 12, 64, 2)
+
+hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s) {
+#define X(RR, LM, GG, SS) \
+  if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS>(a, grid, s);
+  HB_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_count_all(int r, int log2m, const uint32_t *c, uint64_t n, double amm, uint32_t m,
+                            const double *lc, double *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+#define X(RR, LM, GG, SS) \
+  if (r == RR && log2m == LM) return launch_count_t<RR, GG, SS>(c, n, amm, m, lc, out, s);
+  HB_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_seed(const SeedArgs &a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  seed_kernel<<<grid_for(a.n, kBlock, 65536), kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const Partial *p, uint32_t np, const uint32_t *list_len, Stats *out,
+                         hipStream_t s) {
+  reduce_kernel<<<1, kBlock, 0, s>>>(p, np, list_len, out);
+  return hipGetLastError();
+}
+
+uint64_t compact_scratch_words(uint64_t n) {
+  const uint64_t nw = (n + 31) / 32;
+  return (nw + kBlock - 1) / kBlock + 1;
+}
+
+hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *bsum,
+                          uint32_t *list, uint32_t *len, hipStream_t s) {
+  const uint64_t nw = (n + 31) / 32;
+  const uint32_t nb = uint32_t((nw + kBlock - 1) / kBlock);
+  if (nb == 0) {
+    fill_u32_kernel<<<1, 64, 0, s>>>(len, 1, 0);
+    return hipGetLastError();
+  }
+  compact_count_kernel<<<nb, kBlock, 0, s>>>(a, b, n, bsum);
+  compact_scan_kernel<<<1, 1024, 0, s>>>(bsum, nb, len);
+  compact_emit_kernel<<<nb, kBlock, 0, s>>>(a, b, n, bsum, list);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s) {
+  if (words == 0) return hipSuccess;
+  fill_u32_kernel<<<grid_for(words, kBlock, 16384), kBlock, 0, s>>>(p, words, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_bits(uint32_t *p, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_bits_kernel<<<grid_for((n + 31) / 32, kBlock, 16384), kBlock, 0, s>>>(p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_reference_layout(const uint32_t *c, uint64_t node_begin, uint64_t node_count,
+                                      uint32_t r, uint32_t m, uint64_t *out, hipStream_t s) {
+  const uint64_t total = node_count * (uint64_t(m) * r / 64);
+  if (total == 0) return hipSuccess;
+  to_reference_kernel<<<grid_for(total, kBlock, 65536), kBlock, 0, s>>>(c, node_begin, node_count,
+                                                                        r, m, out);
+  return hipGetLastError();
+}
+
+}  // namespace hb

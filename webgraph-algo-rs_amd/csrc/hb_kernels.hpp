@@ -1,0 +1,109 @@
+// Kernel launch interface of the HyperBall library (device side in
+// hb_kernels.hip).
+//
+// Device counter layout ("slab" layout).  A counter of m registers of r bits
+// is m/32 slabs; slab s holds registers 32s..32s+31 as r bit planes of 32
+// bits (dword s*r + b holds bit b of each of the 32 registers, register
+// 32s+q at bit q).  A counter is therefore m*r/32 dwords = exactly the bytes
+// of the reference's packed layout (hyper_log_log_logic.rs:72-115), with the
+// bits permuted so that a per-register maximum is a handful of bitwise
+// operations per dword instead of the reference's broadword borrow chains
+// (hyper_log_log_logic.rs:469-576).  kernel_to_reference_layout converts back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hb {
+
+constexpr int kBlock = 256;
+constexpr int kMaxDiscount = 8;
+
+enum UnionFlags : uint32_t {
+  F_SYSTOLIC = 1u,    // accumulate the neighbourhood function as post - pre; mark predecessors
+  F_CHECK_ALL = 2u,   // every processed node enumerates its successors
+  F_CENTRALITY = 4u,  // update the per-node accumulators
+  F_LIST = 8u,        // nodes come from a compacted list whose length lives on the device
+};
+
+enum AccMode : uint32_t { ACC_F64 = 0, ACC_F32_KAHAN = 1, ACC_F32 = 2 };
+
+// Per-block partial results of one union launch.
+struct Partial {
+  double nf;
+  uint64_t visited, scanned, tscanned;
+  uint32_t checked, copied, modified, pad;
+};
+
+// Reduced results of one iteration.
+struct Stats {
+  double nf;
+  uint64_t visited, scanned, tscanned, checked, copied, modified, list_len;
+};
+
+struct UnionArgs {
+  const uint64_t *off;
+  const uint32_t *succ;
+  const uint64_t *toff;
+  const uint32_t *tsucc;
+  const uint32_t *curr;  // counters, slab layout
+  uint32_t *next;
+  const uint32_t *curr_mod;
+  uint32_t *next_mod;
+  const uint32_t *must;
+  uint32_t *next_must;
+  double *cache;  // count() of every current counter
+  const uint32_t *list;
+  const uint32_t *list_len;
+  uint64_t count;  // number of nodes when not in list mode
+  uint32_t flags;
+  uint32_t acc_mode;
+  void *sd;
+  void *hc;
+  void *dc[kMaxDiscount];
+  double disc[kMaxDiscount];
+  int ndisc;
+  double dist;  // iteration + 1
+  double alpha_m_m;
+  uint32_t m;
+  const double *lc_table;  // m * ln(m / z) for z = 0..m
+  Partial *partials;
+};
+
+struct SeedArgs {
+  uint32_t *counters;  // zeroed, slab layout
+  uint64_t n;
+  uint32_t cdw;        // dwords per counter
+  uint32_t r, log2m;
+  uint64_t sentinel;
+  uint32_t hash;
+  uint64_t seed;
+  const uint64_t *weights;       // nullable
+  const uint64_t *weight_prefix; // exclusive prefix sums of weights
+  uint64_t weight_seed;
+};
+
+// Chooses the lane group of a (r, log2m) shape: lanes per node and slabs per
+// lane.  Returns false for shapes without a kernel.
+bool union_shape(int r, int log2m, int *lanes, int *slabs_per_lane);
+
+hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s);
+hipError_t launch_count_all(int r, int log2m, const uint32_t *counters, uint64_t n,
+                            double alpha_m_m, uint32_t m, const double *lc_table, double *out,
+                            hipStream_t s);
+hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
+hipError_t launch_reduce(const Partial *p, uint32_t np, const uint32_t *list_len, Stats *out,
+                         hipStream_t s);
+// list <- ids of the set bits of (a | b) below n, ascending; *len <- count.
+hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *block_sums,
+                          uint32_t *list, uint32_t *len, hipStream_t s);
+uint64_t compact_scratch_words(uint64_t n);
+hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s);
+// Sets bits [0, n) and clears the tail of the last word.
+hipError_t launch_fill_bits(uint32_t *p, uint64_t n, hipStream_t s);
+hipError_t launch_to_reference_layout(const uint32_t *counters, uint64_t node_begin,
+                                      uint64_t node_count, uint32_t r, uint32_t m,
+                                      uint64_t *out, hipStream_t s);
+
+}  // namespace hb
