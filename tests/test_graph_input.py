@@ -1,0 +1,103 @@
+"""Host-side graph input: the BVGraph decoder against the files' own
+metadata, the transpose, and the synthetic generators."""
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cnr-2000")
+
+
+def _props(name):
+    out = {}
+    for line in open(os.path.join(GOLD, name + ".properties")):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.strip().split("=", 1)
+            out[k] = v
+    return out
+
+
+@pytest.mark.parametrize("name", ["cnr-2000", "cnr-2000-t"])
+def test_bvgraph_decode_matches_metadata(hbm, name):
+    g = hbm.Graph.load_bvgraph(os.path.join(GOLD, name))
+    p = _props(name)
+    assert g.num_nodes == int(p["nodes"]) == 325557
+    assert g.num_arcs == int(p["arcs"]) == 3216152
+    # rows strictly increasing, ids in range
+    d = np.diff(g.successors.astype(np.int64))
+    starts = np.zeros(g.num_arcs, bool)
+    starts[g.offsets[:-1][g.offsets[:-1] < g.num_arcs].astype(np.int64)] = True
+    assert np.all((d > 0) | starts[1:])
+    assert g.successors.max() < g.num_nodes
+
+
+@pytest.mark.parametrize("name", ["cnr-2000", "cnr-2000-t"])
+def test_bvgraph_bit_offsets_match_offsets_file(hbm, name):
+    import ctypes
+    base = os.path.join(GOLD, name).encode()
+    n = int(_props(name)["nodes"])
+    recorded = np.zeros(n + 1, np.uint64)
+    seen = np.zeros(n + 1, np.uint64)
+    hbm._check(hbm.lib().hb_bvgraph_load_offsets(base, n, recorded.ctypes.data))
+    hbm._check(hbm.lib().hb_bvgraph_decoder_offsets(base, seen.ctypes.data, n + 1))
+    assert np.array_equal(recorded, seen)
+
+
+def test_cnr_transpose_consistency(hbm, cnr):
+    g, gt = cnr
+    t = g.transpose()
+    assert np.array_equal(t.offsets, gt.offsets)
+    assert np.array_equal(t.successors, gt.successors)
+    tt = gt.transpose()
+    assert np.array_equal(tt.offsets, g.offsets)
+    assert np.array_equal(tt.successors, g.successors)
+
+
+def test_transpose_brute_force(hbm):
+    rng = np.random.default_rng(3)
+    arcs = rng.integers(0, 50, size=(400, 2))
+    g = hbm.Graph.from_arcs(50, arcs)
+    t = g.transpose()
+    fwd = {(u, int(v)) for u in range(50) for v in g.successors[g.offsets[u]:g.offsets[u + 1]]}
+    bwd = {(int(v), u) for u in range(50) for v in t.successors[t.offsets[u]:t.offsets[u + 1]]}
+    assert fwd == bwd
+
+
+def test_uniform_generator(hbm):
+    g = hbm.Graph.uniform(10000, 8, seed=1)
+    assert g.num_arcs == 80000
+    assert np.all(g.outdegrees() == 8)
+    src = np.repeat(np.arange(10000), 8)
+    assert not np.any(g.successors == src)  # no self loops
+    rows = g.successors.reshape(10000, 8)
+    assert np.all(np.diff(rows.astype(np.int64), axis=1) > 0)  # distinct, sorted
+    g2 = hbm.Graph.uniform(10000, 8, seed=1)
+    assert np.array_equal(g.successors, g2.successors)  # deterministic
+    # roughly uniform in-degree
+    indeg = np.bincount(g.successors, minlength=10000)
+    assert abs(indeg.mean() - 8) < 1e-9 and indeg.std() < 4
+
+
+def test_web_generator(hbm):
+    n = 200000
+    g = hbm.Graph.web(n, 2.1, 3, 10000, 1000, 0.7, 2)
+    deg = g.outdegrees().astype(np.int64)
+    assert deg.max() <= 10000
+    src = np.repeat(np.arange(n), deg)
+    assert not np.any(g.successors == src)
+    d = np.diff(g.successors.astype(np.int64))
+    starts = np.zeros(g.num_arcs, bool)
+    starts[g.offsets[:-1][g.offsets[:-1] < g.num_arcs].astype(np.int64)] = True
+    assert np.all((d > 0) | starts[1:])
+    # about 70% of arcs fall in the window (wrapping distance <= 1000)
+    dist = np.abs(g.successors.astype(np.int64) - src)
+    dist = np.minimum(dist, n - dist)
+    frac = np.mean(dist <= 1000)
+    assert 0.65 < frac < 0.8
+    # preferential attachment makes in-degree heavy tailed
+    indeg = np.bincount(g.successors, minlength=n)
+    assert indeg.max() > 10 * indeg.mean()  # copy model with 30% copying: exponent ~ 1 + 1/0.3
+    # mean out-degree of Zipf(2.1) on [3, 10000] is about 15.7 before dedup
+    assert 12 < g.num_arcs / n < 16.5
+    g2 = hbm.Graph.web(n, 2.1, 3, 10000, 1000, 0.7, 2)
+    assert np.array_equal(g.successors, g2.successors)
