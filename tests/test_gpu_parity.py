@@ -27,11 +27,13 @@ def _tiny_graphs(hbm):
             "self_loops": loops}
 
 
-def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_000):
+def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_000, heavy=None):
     b = hbm.HyperBallBuilder.with_hyper_log_log(g, gt, log2m).register_bits(reg_bits)
     if jenkins is not None:
         b = b.jenkins(jenkins)
     hb = b.build()
+    if heavy is not None:
+        hb.set_tuning(heavy)
     hll = orc.Hll(log2m, reg_bits, g.num_nodes, orc.HASH_JENKINS if jenkins is not None else orc.HASH_SIPHASH13,
                   jenkins or 0)
     ref = orc.Jacobi(hll, g.offsets, g.successors)
@@ -124,3 +126,18 @@ def test_cnr_2000_par_equals_seq_every_iteration(hbm, orc, cnr):
     it, stats = _run_lockstep(hbm, orc, g, gt, 6)
     assert it > 10
     assert any(s["systolic"] for s in stats) and any(s["local"] for s in stats)
+
+
+@pytest.mark.parametrize("heavy", [64, 0, 2**32 - 1])
+@pytest.mark.parametrize("log2m,reg_bits", [(6, 5), (8, 5), (10, 6), (12, 5)])
+def test_heavy_rows_bit_exact(hbm, orc, heavy, log2m, reg_bits):
+    """Rows above the threshold take the workgroup-per-node kernel; results
+    must not depend on the threshold."""
+    rng = np.random.default_rng(log2m)
+    n = 6000
+    arcs = [(0, int(v)) for v in rng.choice(np.arange(1, n), 3000, replace=False)]
+    arcs += [(7, int(v)) for v in rng.choice(np.arange(n), 700, replace=False) if v != 7]
+    arcs += [(int(u), int((u * 7 + 13) % n)) for u in range(n)]
+    arcs += [(int(u), int((u + 1) % n)) for u in range(0, n, 3)]
+    g = hbm.Graph.from_arcs(n, arcs)
+    _run_lockstep(hbm, orc, g, g.transpose(), log2m, reg_bits, jenkins=5, heavy=heavy)

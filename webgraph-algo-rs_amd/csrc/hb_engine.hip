@@ -14,6 +14,7 @@
 // hyperball_impl.rs:1302-1319).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -86,7 +87,17 @@ struct Instance {
   bool weighted = false, has_t = false, do_sd = false, do_hc = false;
   uint32_t acc_mode = ACC_F64;
   std::vector<Discount> discounts;
-  uint32_t heavy_degree = 0;
+  // Rows longer than heavy_degree go to the workgroup-per-node kernel.
+  // heavy_nodes holds every node with more than kHeavyFloor successors,
+  // sorted by decreasing degree, so a threshold selects a prefix.
+  static constexpr uint64_t kHeavyFloor = 64;
+  static constexpr uint64_t kDefaultHeavy = 1024;
+  static constexpr uint32_t kHeavyGridCap = 4096;
+  uint64_t heavy_degree = kDefaultHeavy;
+  std::vector<uint32_t> heavy_nodes;
+  std::vector<uint64_t> heavy_degs;
+  DevBuf<uint32_t> d_heavy;
+  uint64_t heavy_count = 0;
 
   DevBuf<uint64_t> off, toff, d_weights, d_wprefix;
   DevBuf<uint32_t> succ, tsucc;
@@ -126,6 +137,13 @@ struct Instance {
   size_t acc_bytes() const { return acc_mode == ACC_F32 ? 4 : 8; }
   bool do_centrality() const { return do_sd || do_hc || !discounts.empty(); }
   void use_device() const { HB_HIP(hipSetDevice(device)); }
+
+  void set_heavy_degree(uint64_t h) {
+    heavy_degree = std::max(h, kHeavyFloor);
+    uint64_t k = 0;
+    while (k < heavy_degs.size() && heavy_degs[k] > heavy_degree) k++;
+    heavy_count = k;
+  }
 
   void fill_zero(void *p, size_t bytes) {
     HB_HIP(launch_fill_u32(static_cast<uint32_t *>(p), bytes / 4, 0u, stream));
@@ -236,10 +254,16 @@ struct Instance {
     uint32_t grid = max_grid;
     if (!list_mode) grid = uint32_t(std::min<uint64_t>(max_grid, std::max<uint64_t>(8, (n + npb - 1) / npb)));
     grid = (grid + 7) / 8 * 8;
+    const uint32_t heavy_grid = uint32_t(std::min<uint64_t>(heavy_count, kHeavyGridCap));
+    ua.heavy_degree = heavy_count ? heavy_degree : UINT64_MAX;
+    ua.heavy_list = d_heavy.p;
+    ua.heavy_count = heavy_count;
+    ua.heavy_partial_base = grid;
     if (timing) HB_HIP(hipEventRecord(ev[1], stream));
-    HB_HIP(launch_union(int(r), int(log2m), ua, grid, stream));
+    HB_HIP(launch_union(int(r), int(log2m), ua, grid, heavy_grid, stream));
     if (timing) HB_HIP(hipEventRecord(ev[2], stream));
-    HB_HIP(launch_reduce(partials.p, grid, list_mode ? list_len.p : nullptr, stats.p, stream));
+    HB_HIP(launch_reduce(partials.p, grid + heavy_grid, list_mode ? list_len.p : nullptr, stats.p,
+                         stream));
     HB_HIP(hipMemcpyAsync(h_stats, stats.p, sizeof(Stats), hipMemcpyDeviceToHost, stream));
     if (timing) HB_HIP(hipEventRecord(ev[3], stream));
     HB_HIP(hipStreamSynchronize(stream));
@@ -512,7 +536,22 @@ hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_confi
     I.list_len.alloc(1);
     I.max_grid = std::max<uint32_t>(64, I.cus * 8);
     I.max_grid = (I.max_grid + 7) / 8 * 8;
-    I.partials.alloc(I.max_grid);
+    I.partials.alloc(I.max_grid + Instance::kHeavyGridCap);
+    {
+      std::vector<std::pair<uint64_t, uint32_t>> heavy;
+      for (uint64_t v = 0; v < n; v++) {
+        const uint64_t d = graph->offsets[v + 1] - graph->offsets[v];
+        if (d > Instance::kHeavyFloor) heavy.push_back({d, uint32_t(v)});
+      }
+      std::sort(heavy.begin(), heavy.end(),
+                [](const auto &x, const auto &y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });
+      for (auto &h : heavy) {
+        I.heavy_degs.push_back(h.first);
+        I.heavy_nodes.push_back(h.second);
+      }
+      upload(I.d_heavy, I.heavy_nodes.data(), I.heavy_nodes.size(), s);
+      I.set_heavy_degree(Instance::kDefaultHeavy);
+    }
     I.stats.alloc(1);
     HB_HIP(hipStreamSynchronize(s));
     *out = reinterpret_cast<hb_instance *>(inst.release());
@@ -734,7 +773,11 @@ hb_status hb_set_timing(hb_instance *hb, int enabled) {
 }
 
 hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree) {
-  return guard([&] { checked(hb)->heavy_degree = heavy_degree; });
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.set_heavy_degree(heavy_degree == 0 ? Instance::kDefaultHeavy
+                       : heavy_degree == UINT32_MAX ? UINT64_MAX : heavy_degree);
+  });
 }
 
 }  // extern "C"

@@ -260,66 +260,70 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
   return (pb % 8) * (nb / 8) + pb / 8;
 }
 
+// Merges into `cur` the counters of the modified successors (other than v)
+// found at positions first, first + stride, ... of v's row, G ids at a time
+// (hyperball_impl.rs:1037-1049).  `prev` / `cur` are loaded from the node's
+// own counter on the first hit.
 template <int R, int G, int SPL>
-__device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int gl, Acc &acc_out) {
+__device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v, int gl,
+                                                 uint64_t first, uint64_t end, uint64_t stride,
+                                                 uint32_t *prev, uint32_t *cur, bool &loaded,
+                                                 uint32_t &nvisit) {
   constexpr int D = R * SPL;         // dwords per lane
   constexpr int CD = G * D;          // dwords per counter
   constexpr int U = D <= 12 ? 4 : 2; // successor loads in flight per lane
-  const bool systolic = a.flags & F_SYSTOLIC;
-  const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
-  const bool was_mod = test_bit(a.curr_mod, v);
-  if (!check && !was_mod) return;
-
-  uint32_t prev[D], cur[D];
-  bool loaded = false;
-  uint32_t nvisit = 0;
-  uint64_t deg = 0;
   const uint32_t *own = a.curr + uint64_t(v) * CD + gl * D;
-
-  if (check) {
-    const uint64_t beg = a.off[v], end = a.off[v + 1];
-    deg = end - beg;
-    for (uint64_t base = beg; base < end; base += G) {
-      const uint64_t idx = base + gl;
-      const uint32_t s = idx < end ? a.succ[idx] : v;
-      const bool f = (s != v) && test_bit(a.curr_mod, s);
-      uint64_t mask = group_ballot<G>(f);
-      if (!mask) continue;
-      if (!loaded) {
-        load_part<D>(prev, own);
+  for (uint64_t base = first; base < end; base += stride) {
+    const uint64_t idx = base + gl;
+    const uint32_t s = idx < end ? a.succ[idx] : v;
+    const bool f = (s != v) && test_bit(a.curr_mod, s);
+    uint64_t mask = group_ballot<G>(f);
+    if (!mask) continue;
+    if (!loaded) {
+      load_part<D>(prev, own);
 #pragma unroll
-        for (int k = 0; k < D; k++) cur[k] = prev[k];
-        loaded = true;
-      }
-      nvisit += __popcll(mask);
-      while (mask) {
-        uint32_t t[U];
-        int cnt = 0;
+      for (int k = 0; k < D; k++) cur[k] = prev[k];
+      loaded = true;
+    }
+    nvisit += __popcll(mask);
+    while (mask) {
+      uint32_t t[U];
+      int cnt = 0;
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          if (mask) {
-            const int b = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            t[u] = group_shfl<G>(s, b);
-            cnt++;
-          } else {
-            t[u] = 0;
-          }
+      for (int u = 0; u < U; u++) {
+        if (mask) {
+          const int b = __builtin_ctzll(mask);
+          mask &= mask - 1;
+          t[u] = group_shfl<G>(s, b);
+          cnt++;
+        } else {
+          t[u] = 0;
         }
-        uint32_t y[U][D];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-          if (u < cnt) load_part<D>(y[u], a.curr + uint64_t(t[u]) * CD + gl * D);
-#pragma unroll
-        for (int u = 0; u < U; u++)
-          if (u < cnt) {
-#pragma unroll
-            for (int k = 0; k < SPL; k++) slab_max<R>(cur + k * R, y[u] + k * R);
-          }
       }
+      uint32_t y[U][D];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (u < cnt) load_part<D>(y[u], a.curr + uint64_t(t[u]) * CD + gl * D);
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (u < cnt) {
+#pragma unroll
+          for (int k = 0; k < SPL; k++) slab_max<R>(cur + k * R, y[u] + k * R);
+        }
     }
   }
+}
 
+// Everything after the successor loop (hyperball_impl.rs:1051-1144): changed
+// test, count(), neighbourhood function, centralities, modified bit,
+// predecessor marking and the store into the next state.
+template <int R, int G, int SPL>
+__device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int gl, bool check,
+                                            bool was_mod, bool loaded, const uint32_t *prev,
+                                            uint32_t *cur, uint32_t nvisit, uint64_t deg,
+                                            Acc &acc_out) {
+  constexpr int D = R * SPL, CD = G * D;
+  const bool systolic = a.flags & F_SYSTOLIC;
   bool cm = false;
   if (loaded) {
     uint32_t d = 0;
@@ -368,9 +372,26 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int
   }
 
   if (cm || was_mod) {
-    if (!loaded) load_part<D>(cur, own);
+    if (!loaded) load_part<D>(cur, a.curr + uint64_t(v) * CD + gl * D);
     store_part<D>(a.next + uint64_t(v) * CD + gl * D, cur);
   }
+}
+
+template <int R, int G, int SPL>
+__device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int gl, Acc &acc_out) {
+  constexpr int D = R * SPL;
+  const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
+  const bool was_mod = test_bit(a.curr_mod, v);
+  if (!check && !was_mod) return;
+  const uint64_t beg = a.off[v], end = a.off[v + 1];
+  if (end - beg > a.heavy_degree) return;  // handled by heavy_kernel
+
+  uint32_t prev[D], cur[D];
+  bool loaded = false;
+  uint32_t nvisit = 0;
+  if (check) merge_successors<R, G, SPL>(a, v, gl, beg, end, G, prev, cur, loaded, nvisit);
+  finish_node<R, G, SPL>(a, v, gl, check, was_mod, loaded, prev, cur, nvisit,
+                         check ? end - beg : 0, acc_out);
 }
 
 template <int R, int G, int SPL>
@@ -388,6 +409,73 @@ __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
   }
   const Partial p = block_sum(acc);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = p;
+}
+
+// Nodes with more than heavy_degree successors: one workgroup per node.  Its
+// kBlock/G lane groups take interleaved G-wide slices of the row, keep
+// private maxima, and fold them in LDS; lane group 0 finishes the node.
+template <int R, int G, int SPL>
+__global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
+  constexpr int D = R * SPL, NG = kBlock / G;
+  __shared__ uint32_t s_acc[NG][G * D];
+  __shared__ uint32_t s_visit[NG];
+  __shared__ uint32_t s_any;
+  const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  Acc acc;
+  for (uint64_t h = blockIdx.x; h < a.heavy_count; h += gridDim.x) {
+    const uint32_t v = a.heavy_list[h];
+    const bool in_list = !(a.flags & F_LIST) || test_bit(a.must, v) || test_bit(a.curr_mod, v);
+    const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
+    const bool was_mod = test_bit(a.curr_mod, v);
+    if (!in_list || (!check && !was_mod)) continue;  // uniform over the block
+    const uint64_t beg = a.off[v], end = a.off[v + 1];
+    uint32_t prev[D], cur[D];
+    bool loaded = false;
+    uint32_t nvisit = 0;
+    if (check)
+      merge_successors<R, G, SPL>(a, v, gl, beg + uint64_t(g) * G, end, uint64_t(NG) * G, prev,
+                                  cur, loaded, nvisit);
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    if (loaded && gl == 0) atomicOr(&s_any, 1u);
+    if (gl == 0) s_visit[g] = nvisit;
+#pragma unroll
+    for (int k = 0; k < D; k++) s_acc[g][gl * D + k] = loaded ? cur[k] : 0u;
+    __syncthreads();
+    for (int half = NG / 2; half > 0; half >>= 1) {
+      if (g < half) {
+        uint32_t x[D], y[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+          x[k] = s_acc[g][gl * D + k];
+          y[k] = s_acc[g + half][gl * D + k];
+        }
+#pragma unroll
+        for (int k = 0; k < SPL; k++) slab_max<R>(x + k * R, y + k * R);
+#pragma unroll
+        for (int k = 0; k < D; k++) s_acc[g][gl * D + k] = x[k];
+      }
+      __syncthreads();
+    }
+    if (g == 0) {
+      const bool any = s_any != 0;
+      uint32_t total_visit = 0;
+      for (int k = 0; k < NG; k++) total_visit += s_visit[k];
+      if (any) {
+        load_part<D>(prev, a.curr + uint64_t(v) * (G * D) + gl * D);
+#pragma unroll
+        for (int k = 0; k < D; k++) cur[k] = s_acc[0][gl * D + k];
+        // the folded maxima never lose the node's own registers
+#pragma unroll
+        for (int k = 0; k < SPL; k++) slab_max<R>(cur + k * R, prev + k * R);
+      }
+      finish_node<R, G, SPL>(a, v, gl, check, was_mod, any, prev, cur, total_visit,
+                             check ? end - beg : 0, acc);
+    }
+    __syncthreads();
+  }
+  const Partial p = block_sum(acc);
+  if (threadIdx.x == 0) a.partials[a.heavy_partial_base + blockIdx.x] = p;
 }
 
 template <int R, int G, int SPL>
@@ -578,7 +666,8 @@ uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
 }
 
 template <int R, int G, int SPL>
-hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, hipStream_t s) {
+hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, uint32_t heavy_grid, hipStream_t s) {
+  if (heavy_grid) heavy_kernel<R, G, SPL><<<heavy_grid, kBlock, 0, s>>>(a);
   grid = (grid + 7) / 8 * 8;  // xcd_logical_block needs a multiple of 8
   union_kernel<R, G, SPL><<<grid, kBlock, 0, s>>>(a);
   return hipGetLastError();
@@ -608,9 +697,10 @@ bool union_shape(int r, int log2m, int *lanes, int *spl) {
   X(5, 11, 32, 2) X(5, 12, 64, 2) X(6, 5, 1, 1) X(6, 6, 2, 1) X(6, 7, 4, 1) X(6, 8, 8, 1)  \
   X(6, 9, 8, 2) X(6, 10, 16, 2) X(6, 11, 32, 2) X(6, 12, 64, 2)
 
-hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s) {
+hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, uint32_t heavy_grid,
+                        hipStream_t s) {
 #define X(RR, LM, GG, SS) \
-  if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS>(a, grid, s);
+  if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS>(a, grid, heavy_grid, s);
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;

@@ -69,6 +69,10 @@ struct UnionArgs {
   uint32_t m;
   const double *lc_table;  // m * ln(m / z) for z = 0..m
   Partial *partials;
+  uint64_t heavy_degree;         // rows longer than this go to heavy_kernel
+  const uint32_t *heavy_list;    // the nodes with such rows
+  uint64_t heavy_count;
+  uint32_t heavy_partial_base;   // heavy_kernel's partials start here
 };
 
 struct SeedArgs {
@@ -88,7 +92,10 @@ struct SeedArgs {
 // lane.  Returns false for shapes without a kernel.
 bool union_shape(int r, int log2m, int *lanes, int *slabs_per_lane);
 
-hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s);
+// Launches heavy_kernel over a.heavy_list (heavy_grid workgroups, 0 = none)
+// and then union_kernel (grid workgroups).
+hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, uint32_t heavy_grid,
+                        hipStream_t s);
 hipError_t launch_count_all(int r, int log2m, const uint32_t *counters, uint64_t n,
                             double alpha_m_m, uint32_t m, const double *lc_table, double *out,
                             hipStream_t s);

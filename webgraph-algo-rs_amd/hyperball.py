@@ -479,3 +479,8 @@ class HyperBall:
 
     def set_timing(self, on: bool) -> None:
         _check(lib().hb_set_timing(self._h, int(on)))
+
+    def set_tuning(self, heavy_degree: int) -> None:
+        """Rows longer than heavy_degree go to the workgroup-per-node kernel
+        (0 = default, 2**32 - 1 = never)."""
+        _check(lib().hb_set_tuning(self._h, int(heavy_degree)))
