@@ -83,12 +83,26 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stats", action="store_true", help="print per-iteration statistics to stderr")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (a known-size "
+                         "kernel that scales profiler byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is "
                                    "designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.calibrate_copy_mib:
+        import torch
+        nbytes = args.calibrate_copy_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(f"# calibration copy of {args.calibrate_copy_mib} MiB done", file=sys.stderr, flush=True)
 
     hbm = _load("hyperball", os.path.join(PKG, "hyperball.py"))
     if not os.path.exists(hbm.LIB_PATH):
