@@ -78,14 +78,15 @@ def test_web_systolic_local_bit_exact(hbm, orc, hash_):
     assert any(m[0] for m in modes), "no systolic iteration was exercised"
 
 
-def test_engine_outputs_match_reference_engine(hbm, orc):
+@pytest.mark.parametrize("log2m,reg_bits", [(8, 5), (6, 6), (10, 6), (12, 6), (11, 5)])
+def test_engine_outputs_match_reference_engine(hbm, orc, log2m, reg_bits):
     g = hbm.Graph.web(30000, dmin=1, dmax=300, window=200, seed=9)
     gt = g.transpose()
-    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).jenkins(3)
+    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, log2m).register_bits(reg_bits).jenkins(3)
           .sum_of_distances(True).sum_of_inverse_distances(True).build())
     hb.run_until_done()
-    ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=8,
-                              hash=orc.HASH_JENKINS, seed=3, sum_of_distances=True,
+    ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=log2m,
+                              reg_bits=reg_bits, hash=orc.HASH_JENKINS, seed=3, sum_of_distances=True,
                               sum_of_inverse_distances=True)
     ref.run()
     assert np.array_equal(hb.counters(0), ref.counters(0))

@@ -91,7 +91,7 @@ struct Instance {
   // heavy_nodes holds every node with more than kHeavyFloor successors,
   // sorted by decreasing degree, so a threshold selects a prefix.
   static constexpr uint64_t kHeavyFloor = 64;
-  static constexpr uint64_t kDefaultHeavy = 1024;
+  static constexpr uint64_t kDefaultHeavy = 512;
   static constexpr uint32_t kHeavyGridCap = 4096;
   uint64_t heavy_degree = kDefaultHeavy;
   std::vector<uint32_t> heavy_nodes;
@@ -99,6 +99,12 @@ struct Instance {
   DevBuf<uint32_t> d_heavy;
   uint64_t heavy_count = 0;
 
+  // Dense iterations visit nodes in `order`: ids sorted by decreasing
+  // out-degree inside consecutive windows of kOrderWindow nodes, so the lane
+  // groups of a wavefront get rows of similar length while successor reads
+  // keep the locality of the id order.
+  static constexpr uint64_t kOrderWindow = 256;
+  DevBuf<uint32_t> d_order;
   DevBuf<uint64_t> off, toff, d_weights, d_wprefix;
   DevBuf<uint32_t> succ, tsucc;
   DevBuf<uint32_t> cnt[2], mod[2], must[2];
@@ -106,8 +112,8 @@ struct Instance {
   DevBuf<uint8_t> sd, hc;
   std::vector<std::unique_ptr<DevBuf<uint8_t>>> dc;
   DevBuf<uint32_t> list, bsum, list_len;
-  DevBuf<Partial> partials;
   DevBuf<Stats> stats;
+  DevBuf<uint32_t> work;
   Stats *h_stats = nullptr;
   uint32_t max_grid = 0;
   uint64_t bitmap_words = 0;  // u32 words, even
@@ -232,6 +238,7 @@ struct Instance {
     ua.cache = cache.p;
     ua.list = list.p;
     ua.list_len = list_len.p;
+    ua.order = d_order.p;
     ua.count = n;
     ua.flags = flags;
     ua.acc_mode = acc_mode;
@@ -246,7 +253,8 @@ struct Instance {
     ua.alpha_m_m = amm;
     ua.m = m;
     ua.lc_table = lc.p;
-    ua.partials = partials.p;
+    ua.stats = stats.p;
+    ua.work = work.p;
 
     int lanes = 1, spl = 1;
     union_shape(int(r), int(log2m), &lanes, &spl);
@@ -258,12 +266,10 @@ struct Instance {
     ua.heavy_degree = heavy_count ? heavy_degree : UINT64_MAX;
     ua.heavy_list = d_heavy.p;
     ua.heavy_count = heavy_count;
-    ua.heavy_partial_base = grid;
+    HB_HIP(launch_reset_stats(stats.p, work.p, stream));
     if (timing) HB_HIP(hipEventRecord(ev[1], stream));
     HB_HIP(launch_union(int(r), int(log2m), ua, grid, heavy_grid, stream));
     if (timing) HB_HIP(hipEventRecord(ev[2], stream));
-    HB_HIP(launch_reduce(partials.p, grid + heavy_grid, list_mode ? list_len.p : nullptr, stats.p,
-                         stream));
     HB_HIP(hipMemcpyAsync(h_stats, stats.p, sizeof(Stats), hipMemcpyDeviceToHost, stream));
     if (timing) HB_HIP(hipEventRecord(ev[3], stream));
     HB_HIP(hipStreamSynchronize(stream));
@@ -291,7 +297,7 @@ struct Instance {
     cur = 1 - cur;
     mcur = 1 - mcur;
     if (systolic) mustcur = 1 - mustcur;
-    double current = base_nf + st.nf;
+    double current = base_nf + stats_nf(st);
     last = current;
     const double last_output = nf.back();
     if (current < last_output) current = last_output;
@@ -518,6 +524,24 @@ hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_confi
       upload(I.d_weights, cfg->weights, n, s);
       upload(I.d_wprefix, prefix.data(), n + 1, s);
     }
+    {
+      std::vector<uint32_t> order(n);
+      parallel_for((n + Instance::kOrderWindow - 1) / Instance::kOrderWindow,
+                   [&](uint64_t b, uint64_t e, unsigned) {
+                     for (uint64_t w = b; w < e; w++) {
+                       const uint64_t lo = w * Instance::kOrderWindow;
+                       const uint64_t hi = std::min(n, lo + Instance::kOrderWindow);
+                       for (uint64_t v = lo; v < hi; v++) order[v] = uint32_t(v);
+                       std::stable_sort(order.begin() + lo, order.begin() + hi,
+                                        [&](uint32_t x, uint32_t y) {
+                                          return graph->offsets[x + 1] - graph->offsets[x] >
+                                                 graph->offsets[y + 1] - graph->offsets[y];
+                                        });
+                     }
+                   });
+      upload(I.d_order, order.data(), n, s);
+      HB_HIP(hipStreamSynchronize(s));
+    }
     std::vector<double> lc(m + 1, 0.0);
     for (uint64_t z = 1; z <= m; z++) lc[z] = double(m) * std::log(double(m) / double(z));
     upload(I.lc, lc.data(), m + 1, s);
@@ -536,7 +560,6 @@ hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_confi
     I.list_len.alloc(1);
     I.max_grid = std::max<uint32_t>(64, I.cus * 8);
     I.max_grid = (I.max_grid + 7) / 8 * 8;
-    I.partials.alloc(I.max_grid + Instance::kHeavyGridCap);
     {
       std::vector<std::pair<uint64_t, uint32_t>> heavy;
       for (uint64_t v = 0; v < n; v++) {
@@ -553,6 +576,7 @@ hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_confi
       I.set_heavy_degree(Instance::kDefaultHeavy);
     }
     I.stats.alloc(1);
+    I.work.alloc(kWorkCounters * kWorkStride);
     HB_HIP(hipStreamSynchronize(s));
     *out = reinterpret_cast<hb_instance *>(inst.release());
   });

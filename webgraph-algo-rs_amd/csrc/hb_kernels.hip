@@ -61,20 +61,23 @@ __device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {
 }
 
 // Per-register maximum of one slab: x = max(x, y) over 32 registers held as
-// R bit planes.  gt marks the registers where x > y, decided from the top
-// plane down.
+// R bit planes.  gt marks the registers where x > y and eq those still tied,
+// decided from the top plane down; d = x ^ y serves the tie update and the
+// final select y ^ (d & gt).
 template <int R>
 __device__ __forceinline__ void slab_max(uint32_t *x, const uint32_t *y) {
-  uint32_t gt = x[R - 1] & ~y[R - 1];
-  uint32_t eq = ~(x[R - 1] ^ y[R - 1]);
+  uint32_t d[R];
 #pragma unroll
-  for (int b = R - 2; b >= 1; b--) {
-    gt |= eq & x[b] & ~y[b];
-    eq &= ~(x[b] ^ y[b]);
+  for (int b = 0; b < R; b++) d[b] = x[b] ^ y[b];
+  uint32_t gt = x[R - 1] & d[R - 1];
+  uint32_t eq = ~d[R - 1];
+#pragma unroll
+  for (int b = R - 2; b >= 0; b--) {
+    gt |= eq & x[b] & d[b];
+    eq &= ~d[b];
   }
-  gt |= eq & x[0] & ~y[0];
 #pragma unroll
-  for (int b = 0; b < R; b++) x[b] = (x[b] & gt) | (y[b] & ~gt);
+  for (int b = 0; b < R; b++) x[b] = y[b] ^ (d[b] & gt);
 }
 
 // count() pieces of one slab.
@@ -109,28 +112,29 @@ __device__ __forceinline__ SlabSum5 slab_sum5(const uint32_t *p) {
 }
 
 struct SlabSum6 {
-  unsigned __int128 s;
+  uint64_t hi;  // sum of 2^(11 - v) over registers with v < 12
+  uint64_t lo;  // sum of 2^(63 - v) over registers with v >= 12
   uint32_t zeros;
 };
 
+// S = hi * 2^52 + lo.  For up to 2^12 registers hi < 2^24 and lo <= 2^63,
+// so neither part overflows 64 bits.
 __device__ __forceinline__ SlabSum6 slab_sum6(const uint32_t *p) {
   SlabSum6 out;
   out.zeros = __popc(~(p[0] | p[1] | p[2] | p[3] | p[4] | p[5]));
   const uint32_t n[6] = {~p[0], ~p[1], ~p[2], ~p[3], ~p[4], ~p[5]};
-  uint32_t q[16];
+  const uint32_t q45[4] = {n[5] & n[4], n[5] & p[4], p[5] & n[4], p[5] & p[4]};
+  uint64_t hi = 0, lo = 0;
 #pragma unroll
   for (int h = 0; h < 16; h++) {
-    q[h] = ((h & 8) ? p[5] : n[5]) & ((h & 4) ? p[4] : n[4]) & ((h & 2) ? p[3] : n[3]) &
-           ((h & 1) ? p[2] : n[2]);
-  }
-  unsigned __int128 s = 0;
-#pragma unroll
-  for (int h = 0; h < 16; h++) {
-    const uint32_t a = q[h] & n[1], b = q[h] & p[1];
+    const uint32_t qh = q45[h >> 2] & ((h & 2) ? p[3] : n[3]) & ((h & 1) ? p[2] : n[2]);
+    const uint32_t a = qh & n[1], b = qh & p[1];
     const uint32_t t = 8 * __popc(a) - 4 * __popc(a & p[0]) + 2 * __popc(b) - __popc(b & p[0]);
-    s += (unsigned __int128)t << (60 - 4 * h);
+    if (h < 3) hi += uint64_t(t) << (8 - 4 * h);
+    else lo += uint64_t(t) << (60 - 4 * h);
   }
-  out.s = s;
+  out.hi = hi;
+  out.lo = lo;
   return out;
 }
 
@@ -153,22 +157,20 @@ __device__ __forceinline__ double group_count(const uint32_t *x, double alpha_m_
     zeros = group_sum_u32<G>(zeros);
     hm = double(s) * 0x1p-31;
   } else {
-    unsigned __int128 s = 0;
+    uint64_t shi = 0, slo = 0;
 #pragma unroll
     for (int k = 0; k < SPL; k++) {
       SlabSum6 t = slab_sum6(x + k * R);
-      s += t.s;
+      shi += t.hi;
+      slo += t.lo;
       zeros += t.zeros;
     }
-    uint64_t lo = uint64_t(s), hi = uint64_t(s >> 64);
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-      const uint64_t olo = __shfl_xor(lo, o, G), ohi = __shfl_xor(hi, o, G);
-      const uint64_t nlo = lo + olo;
-      hi = hi + ohi + (nlo < lo ? 1 : 0);
-      lo = nlo;
-    }
+    shi = group_sum_u64<G>(shi);
+    slo = group_sum_u64<G>(slo);
     zeros = group_sum_u32<G>(zeros);
+    // S = shi * 2^52 + slo as a 128-bit integer (hi:lo words)
+    const uint64_t lo = slo + (shi << 52);
+    const uint64_t hi = (shi >> 12) + (lo < slo ? 1 : 0);
     hm = (double(hi) * 18446744073709551616.0 + double(lo)) * 0x1p-63;
   }
   double est = alpha_m_m / hm;
@@ -189,9 +191,19 @@ __device__ __forceinline__ void store_part(uint32_t *dst, const uint32_t *src) {
 }
 
 struct Acc {
-  double nf = 0.0;
+  int64_t nf_int = 0, nf_hi = 0, nf_lo = 0;
   uint64_t visited = 0, scanned = 0, tscanned = 0;
   uint32_t checked = 0, copied = 0, modified = 0;
+
+  // Every step is exact except the final rint (|error| <= 2^-65).
+  __device__ __forceinline__ void add_nf(double x) {
+    const double w = floor(x);
+    const double f = (x - w) * 4294967296.0;
+    const double h = floor(f);
+    nf_int += int64_t(w);
+    nf_hi += int64_t(h);
+    nf_lo += int64_t(rint((f - h) * 4294967296.0));
+  }
 };
 
 __device__ __forceinline__ void kahan_add(float2 *p, float x) {
@@ -215,49 +227,43 @@ __device__ __forceinline__ void acc_add(void *base, uint32_t mode, uint32_t v, d
   }
 }
 
-// Block-wide sum of the per-thread accumulators; the result is valid in
-// thread 0.  Every thread of the block must call it.
-__device__ Partial block_sum(const Acc &a) {
-  __shared__ double s_nf[kBlock / 64];
-  __shared__ uint64_t s_u[3][kBlock / 64];
-  __shared__ uint32_t s_c[3][kBlock / 64];
-  double nf = a.nf;
-  uint64_t u0 = a.visited, u1 = a.scanned, u2 = a.tscanned;
-  uint32_t c0 = a.checked, c1 = a.copied, c2 = a.modified;
+// Wavefront sum of the per-lane accumulators, added to *st by lane 0.  Every
+// lane of the wavefront must call it.
+__device__ void flush_stats(const Acc &a, Stats *st) {
+  uint64_t v[9] = {uint64_t(a.nf_int), uint64_t(a.nf_hi), uint64_t(a.nf_lo), a.visited, a.scanned,
+                   a.tscanned, a.checked, a.copied, a.modified};
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    nf += __shfl_xor(nf, o);
-    u0 += __shfl_xor(u0, o);
-    u1 += __shfl_xor(u1, o);
-    u2 += __shfl_xor(u2, o);
-    c0 += __shfl_xor(c0, o);
-    c1 += __shfl_xor(c1, o);
-    c2 += __shfl_xor(c2, o);
-  }
-  const int w = threadIdx.x / 64;
+  for (int k = 0; k < 9; k++)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
   if (__lane_id() == 0) {
-    s_nf[w] = nf;
-    s_u[0][w] = u0; s_u[1][w] = u1; s_u[2][w] = u2;
-    s_c[0][w] = c0; s_c[1][w] = c1; s_c[2][w] = c2;
+    unsigned long long *p = reinterpret_cast<unsigned long long *>(st);
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      if (v[k]) atomicAdd(p + k, (unsigned long long)v[k]);
   }
-  __syncthreads();
-  Partial p{};
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < kBlock / 64; k++) {
-      p.nf += s_nf[k];
-      p.visited += s_u[0][k]; p.scanned += s_u[1][k]; p.tscanned += s_u[2][k];
-      p.checked += s_c[0][k]; p.copied += s_c[1][k]; p.modified += s_c[2][k];
-    }
-  }
-  return p;
 }
 
-// Physical workgroup ids are dealt round-robin to the 8 XCDs; remap them so
-// that each XCD walks a contiguous range of node batches in every round and
-// window-local successor reads hit that XCD's L2.
-__device__ __forceinline__ uint32_t xcd_logical_block() {
-  const uint32_t nb = gridDim.x, pb = blockIdx.x;
-  return (pb % 8) * (nb / 8) + pb / 8;
+// Dynamic work: chunks of consecutive positions are handed out through eight
+// counters, one per XCD (workgroups are dealt round-robin to the 8 XCDs, so
+// blockIdx.x % 8 names the XCD a workgroup runs on).  Each XCD walks its own
+// contiguous eighth of the range, which keeps window-local successor reads in
+// that XCD's L2, and steals from the other ranges when its own is exhausted.
+// Returns UINT64_MAX when no work is left.  Called by one lane.
+__device__ __forceinline__ uint64_t grab_chunk(uint32_t *work, uint64_t chunks) {
+  const uint64_t per = (chunks + 7) / 8;
+  const uint32_t home = blockIdx.x % 8;
+  for (uint32_t k = 0; k < 8; k++) {
+    const uint32_t q = (home + k) % 8;
+    const uint64_t base = q * per;
+    if (base >= chunks) continue;
+    const uint64_t lim = per < chunks - base ? per : chunks - base;
+    uint32_t *ctr = work + q * kWorkStride;
+    if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= lim) continue;
+    const uint32_t i = atomicAdd(ctr, 1u);
+    if (i < lim) return base + i;
+  }
+  return UINT64_MAX;
 }
 
 // Merges into `cur` the counters of the modified successors (other than v)
@@ -271,7 +277,11 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
                                                  uint32_t &nvisit) {
   constexpr int D = R * SPL;         // dwords per lane
   constexpr int CD = G * D;          // dwords per counter
+#ifdef HB_LOADS_IN_FLIGHT
+  constexpr int U = HB_LOADS_IN_FLIGHT;
+#else
   constexpr int U = D <= 12 ? 4 : 2; // successor loads in flight per lane
+#endif
   const uint32_t *own = a.curr + uint64_t(v) * CD + gl * D;
   for (uint64_t base = first; base < end; base += stride) {
     const uint64_t idx = base + gl;
@@ -342,9 +352,9 @@ __device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int 
     } else if (!systolic && check) {
       post = a.cache[v];
     }
-    if (!systolic && check) acc_out.nf += post;
+    if (!systolic && check) acc_out.add_nf(post);
     if (cm) {
-      if (systolic) acc_out.nf += post - pre;
+      if (systolic) acc_out.add_nf(post - pre);
       if (a.flags & F_CENTRALITY) {
         const double delta = post - pre;
         if (delta > 0.0) {
@@ -394,21 +404,34 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int
                          check ? end - beg : 0, acc_out);
 }
 
+// Each wavefront takes chunks of kRoundsPerChunk * (64 / G) positions and
+// gives every lane group one node per round.
+constexpr int kRoundsPerChunk = 16;
+
 template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
-  constexpr int NPB = kBlock / G;  // nodes per block and batch
-  const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  constexpr int NPW = 64 / G;  // nodes per wavefront and round
+  constexpr int CH = kRoundsPerChunk * NPW;
+  const int g = __lane_id() / G, gl = __lane_id() % G;
   const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
+  const uint64_t chunks = (count + CH - 1) / CH;
   Acc acc;
-  for (uint64_t batch = xcd_logical_block(); batch * NPB < count; batch += gridDim.x) {
-    const uint64_t i = batch * NPB + g;
-    if (i < count) {
-      const uint32_t v = (a.flags & F_LIST) ? a.list[i] : uint32_t(i);
-      process_node<R, G, SPL>(a, v, gl, acc);
+  for (;;) {
+    uint64_t c = 0;
+    if (__lane_id() == 0) c = grab_chunk(a.work, chunks);
+    c = __shfl(c, 0);
+    if (c == UINT64_MAX) break;
+#pragma unroll 1
+    for (int round = 0; round < kRoundsPerChunk; round++) {
+      const uint64_t i = c * CH + uint64_t(round) * NPW + g;
+      if (i < count) {
+        const uint32_t v = (a.flags & F_LIST) ? a.list[i] : (a.order ? a.order[i] : uint32_t(i));
+        process_node<R, G, SPL>(a, v, gl, acc);
+      }
     }
   }
-  const Partial p = block_sum(acc);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = p;
+  flush_stats(acc, a.stats);
 }
 
 // Nodes with more than heavy_degree successors: one workgroup per node.  Its
@@ -421,8 +444,14 @@ __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
   __shared__ uint32_t s_visit[NG];
   __shared__ uint32_t s_any;
   const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  __shared__ uint64_t s_h;
   Acc acc;
-  for (uint64_t h = blockIdx.x; h < a.heavy_count; h += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) s_h = atomicAdd(a.work + 8 * kWorkStride, 1u);
+    __syncthreads();
+    const uint64_t h = s_h;
+    __syncthreads();
+    if (h >= a.heavy_count) break;
     const uint32_t v = a.heavy_list[h];
     const bool in_list = !(a.flags & F_LIST) || test_bit(a.must, v) || test_bit(a.curr_mod, v);
     const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
@@ -474,8 +503,7 @@ __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
     }
     __syncthreads();
   }
-  const Partial p = block_sum(acc);
-  if (threadIdx.x == 0) a.partials[a.heavy_partial_base + blockIdx.x] = p;
+  flush_stats(acc, a.stats);
 }
 
 template <int R, int G, int SPL>
@@ -519,25 +547,10 @@ __global__ __launch_bounds__(kBlock) void seed_kernel(SeedArgs a) {
   }
 }
 
-// Deterministic sum of the block partials.
-__global__ __launch_bounds__(kBlock) void reduce_kernel(const Partial *p, uint32_t np,
-                                                        const uint32_t *list_len, Stats *out) {
-  Acc a;
-  for (uint32_t i = threadIdx.x; i < np; i += kBlock) {
-    const Partial q = p[i];
-    a.nf += q.nf;
-    a.visited += q.visited; a.scanned += q.scanned; a.tscanned += q.tscanned;
-    a.checked += q.checked; a.copied += q.copied; a.modified += q.modified;
-  }
-  const Partial t = block_sum(a);
-  if (threadIdx.x == 0) {
-    Stats s;
-    s.nf = t.nf;
-    s.visited = t.visited; s.scanned = t.scanned; s.tscanned = t.tscanned;
-    s.checked = t.checked; s.copied = t.copied; s.modified = t.modified;
-    s.list_len = list_len ? *list_len : 0;
-    *out = s;
-  }
+__global__ void reset_stats_kernel(Stats *st, uint32_t *work) {
+  uint32_t *p = reinterpret_cast<uint32_t *>(st);
+  for (uint32_t i = threadIdx.x; i < sizeof(Stats) / 4; i += blockDim.x) p[i] = 0;
+  for (uint32_t i = threadIdx.x; i < kWorkCounters * kWorkStride; i += blockDim.x) work[i] = 0;
 }
 
 // ---- bitmap compaction ---------------------------------------------------
@@ -668,7 +681,6 @@ uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
 template <int R, int G, int SPL>
 hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, uint32_t heavy_grid, hipStream_t s) {
   if (heavy_grid) heavy_kernel<R, G, SPL><<<heavy_grid, kBlock, 0, s>>>(a);
-  grid = (grid + 7) / 8 * 8;  // xcd_logical_block needs a multiple of 8
   union_kernel<R, G, SPL><<<grid, kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
@@ -682,20 +694,34 @@ hipError_t launch_count_t(const uint32_t *c, uint64_t n, double amm, uint32_t m,
 
 }  // namespace
 
-// Lane group per (r, log2m): G lanes per node, SPL slabs per lane.
-bool union_shape(int r, int log2m, int *lanes, int *spl) {
-  if ((r != 5 && r != 6) || log2m < 5 || log2m > 12) return false;
-  static const int G[8] = {1, 2, 4, 8, 8, 16, 32, 64};
-  static const int S[8] = {1, 1, 1, 1, 2, 2, 2, 2};
-  *lanes = G[log2m - 5];
-  *spl = S[log2m - 5];
-  return true;
-}
+// Lane group per (r, log2m): G lanes per node and SPL slabs per lane, with
+// G * SPL = m / 32.  HB_G_<log2m> / HB_SPL_<log2m> override a row at build
+// time (the tuning sweeps use this).
+#ifndef HB_G_10
+#define HB_G_10 16
+#define HB_SPL_10 2
+#endif
+#ifndef HB_G_8
+#define HB_G_8 8
+#define HB_SPL_8 1
+#endif
+#define HB_SHAPES(X)                                                                           \
+  X(5, 5, 1, 1) X(5, 6, 2, 1) X(5, 7, 4, 1) X(5, 8, HB_G_8, HB_SPL_8) X(5, 9, 8, 2)             \
+  X(5, 10, HB_G_10, HB_SPL_10) X(5, 11, 32, 2) X(5, 12, 64, 2) X(6, 5, 1, 1) X(6, 6, 2, 1)       \
+  X(6, 7, 4, 1) X(6, 8, HB_G_8, HB_SPL_8) X(6, 9, 8, 2) X(6, 10, HB_G_10, HB_SPL_10)             \
+  X(6, 11, 32, 2) X(6, 12, 64, 2)
 
-#define HB_SHAPES(X)                                                                    \
-  X(5, 5, 1, 1) X(5, 6, 2, 1) X(5, 7, 4, 1) X(5, 8, 8, 1) X(5, 9, 8, 2) X(5, 10, 16, 2) \
-  X(5, 11, 32, 2) X(5, 12, 64, 2) X(6, 5, 1, 1) X(6, 6, 2, 1) X(6, 7, 4, 1) X(6, 8, 8, 1)  \
-  X(6, 9, 8, 2) X(6, 10, 16, 2) X(6, 11, 32, 2) X(6, 12, 64, 2)
+bool union_shape(int r, int log2m, int *lanes, int *spl) {
+#define X(RR, LM, GG, SS)  \
+  if (r == RR && log2m == LM) { \
+    *lanes = GG;                \
+    *spl = SS;                  \
+    return true;                \
+  }
+  HB_SHAPES(X)
+#undef X
+  return false;
+}
 
 hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, uint32_t heavy_grid,
                         hipStream_t s) {
@@ -722,9 +748,8 @@ hipError_t launch_seed(const SeedArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_reduce(const Partial *p, uint32_t np, const uint32_t *list_len, Stats *out,
-                         hipStream_t s) {
-  reduce_kernel<<<1, kBlock, 0, s>>>(p, np, list_len, out);
+hipError_t launch_reset_stats(Stats *stats, uint32_t *work, hipStream_t s) {
+  reset_stats_kernel<<<1, 256, 0, s>>>(stats, work);
   return hipGetLastError();
 }
 

@@ -29,18 +29,30 @@ enum UnionFlags : uint32_t {
 
 enum AccMode : uint32_t { ACC_F64 = 0, ACC_F32_KAHAN = 1, ACC_F32 = 2 };
 
-// Per-block partial results of one union launch.
-struct Partial {
-  double nf;
-  uint64_t visited, scanned, tscanned;
-  uint32_t checked, copied, modified, pad;
+// Results of one iteration, accumulated with integer atomics so that they do
+// not depend on the schedule.  The neighbourhood-function contribution is a
+// fixed-point sum with 2^-64 resolution: a term x adds w = floor(x) to nf_int,
+// the first 32 fraction bits of x - w to nf_hi and the next 32 (rounded) to
+// nf_lo.  Integer sums are exact and order-independent, so the value is
+// deterministic run to run and within n * 2^-65 of the exact sum.
+struct Stats {
+  int64_t nf_int, nf_hi, nf_lo;
+  uint64_t visited, scanned, tscanned, checked, copied, modified;
+  uint64_t list_len;
 };
 
-// Reduced results of one iteration.
-struct Stats {
-  double nf;
-  uint64_t visited, scanned, tscanned, checked, copied, modified, list_len;
-};
+// Dynamic work counters, each on its own 256-byte line: [0, 8) one per XCD,
+// [8] heavy rows.
+constexpr int kWorkStride = 64;
+constexpr int kWorkCounters = 9;
+
+inline double stats_nf(const Stats &s) {
+  const int64_t hi = s.nf_hi + (s.nf_lo >> 32);
+  const int64_t lo = s.nf_lo & 0xffffffffll;
+  const int64_t whole = s.nf_int + (hi >> 32);
+  const int64_t frac = hi & 0xffffffffll;
+  return double(whole) + (double(frac) + double(lo) * 0x1p-32) * 0x1p-32;
+}
 
 struct UnionArgs {
   const uint64_t *off;
@@ -56,6 +68,7 @@ struct UnionArgs {
   double *cache;  // count() of every current counter
   const uint32_t *list;
   const uint32_t *list_len;
+  const uint32_t *order;  // processing order when not in list mode (nullable: identity)
   uint64_t count;  // number of nodes when not in list mode
   uint32_t flags;
   uint32_t acc_mode;
@@ -68,11 +81,11 @@ struct UnionArgs {
   double alpha_m_m;
   uint32_t m;
   const double *lc_table;  // m * ln(m / z) for z = 0..m
-  Partial *partials;
+  Stats *stats;            // zeroed before the launch
+  uint32_t *work;          // kWorkCounters * kWorkStride words, zeroed before the launch
   uint64_t heavy_degree;         // rows longer than this go to heavy_kernel
   const uint32_t *heavy_list;    // the nodes with such rows
   uint64_t heavy_count;
-  uint32_t heavy_partial_base;   // heavy_kernel's partials start here
 };
 
 struct SeedArgs {
@@ -100,8 +113,9 @@ hipError_t launch_count_all(int r, int log2m, const uint32_t *counters, uint64_t
                             double alpha_m_m, uint32_t m, const double *lc_table, double *out,
                             hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
-hipError_t launch_reduce(const Partial *p, uint32_t np, const uint32_t *list_len, Stats *out,
-                         hipStream_t s);
+// Zeroes *stats and the work counters (a kernel, not a memset, so the
+// sequence can be captured in a graph).
+hipError_t launch_reset_stats(Stats *stats, uint32_t *work, hipStream_t s);
 // list <- ids of the set bits of (a | b) below n, ascending; *len <- count.
 hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *block_sums,
                           uint32_t *list, uint32_t *len, hipStream_t s);
