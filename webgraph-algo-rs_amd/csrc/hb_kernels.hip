@@ -41,12 +41,6 @@ __device__ __forceinline__ uint64_t group_ballot(bool pred) {
 }
 
 template <int G>
-__device__ __forceinline__ uint32_t group_shfl(uint32_t v, int src) {
-  if constexpr (G == 1) return v;
-  else return __shfl(v, src, G);
-}
-
-template <int G>
 __device__ __forceinline__ uint64_t group_sum_u64(uint64_t v) {
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
@@ -269,12 +263,16 @@ __device__ __forceinline__ uint64_t grab_chunk(uint32_t *work, uint64_t chunks) 
 // Merges into `cur` the counters of the modified successors (other than v)
 // found at positions first, first + stride, ... of v's row, G ids at a time
 // (hyperball_impl.rs:1037-1049).  `prev` / `cur` are loaded from the node's
-// own counter on the first hit.
+// own counter on the first hit.  The ids of the modified successors of each
+// G-wide slice are packed, in ballot order, into the lane group's LDS slot
+// `ids` (G words); they are then merged U at a time with no per-slot
+// predication: a short last batch repeats its first id, and merging a counter
+// twice changes nothing.
 template <int R, int G, int SPL>
 __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v, int gl,
                                                  uint64_t first, uint64_t end, uint64_t stride,
                                                  uint32_t *prev, uint32_t *cur, bool &loaded,
-                                                 uint32_t &nvisit) {
+                                                 uint32_t &nvisit, uint32_t *ids) {
   constexpr int D = R * SPL;         // dwords per lane
   constexpr int CD = G * D;          // dwords per counter
 #ifdef HB_LOADS_IN_FLIGHT
@@ -283,44 +281,39 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
   constexpr int U = D <= 12 ? 4 : 2; // successor loads in flight per lane
 #endif
   const uint32_t *own = a.curr + uint64_t(v) * CD + gl * D;
+  const uint32_t *lane_base = a.curr + gl * D;
   for (uint64_t base = first; base < end; base += stride) {
     const uint64_t idx = base + gl;
     const uint32_t s = idx < end ? a.succ[idx] : v;
     const bool f = (s != v) && test_bit(a.curr_mod, s);
-    uint64_t mask = group_ballot<G>(f);
+    const uint64_t mask = group_ballot<G>(f);
     if (!mask) continue;
+    if (f) ids[__popcll(mask & ((uint64_t(1) << gl) - 1))] = s;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (!loaded) {
       load_part<D>(prev, own);
 #pragma unroll
       for (int k = 0; k < D; k++) cur[k] = prev[k];
       loaded = true;
     }
-    nvisit += __popcll(mask);
-    while (mask) {
-      uint32_t t[U];
-      int cnt = 0;
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (mask) {
-          const int b = __builtin_ctzll(mask);
-          mask &= mask - 1;
-          t[u] = group_shfl<G>(s, b);
-          cnt++;
-        } else {
-          t[u] = 0;
-        }
-      }
+    const uint32_t cnt = __popcll(mask);
+    nvisit += cnt;
+    for (uint32_t k = 0; k < cnt; k += U) {
       uint32_t y[U][D];
 #pragma unroll
-      for (int u = 0; u < U; u++)
-        if (u < cnt) load_part<D>(y[u], a.curr + uint64_t(t[u]) * CD + gl * D);
+      for (int u = 0; u < U; u++) {
+        const uint32_t t = ids[k + u < cnt ? k + u : k];
+        load_part<D>(y[u], lane_base + uint64_t(t) * CD);
+      }
 #pragma unroll
       for (int u = 0; u < U; u++)
-        if (u < cnt) {
 #pragma unroll
-          for (int k = 0; k < SPL; k++) slab_max<R>(cur + k * R, y[u] + k * R);
-        }
+        for (int q = 0; q < SPL; q++) slab_max<R>(cur + q * R, y[u] + q * R);
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -388,7 +381,8 @@ __device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int 
 }
 
 template <int R, int G, int SPL>
-__device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int gl, Acc &acc_out) {
+__device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int gl, Acc &acc_out,
+                                             uint32_t *ids) {
   constexpr int D = R * SPL;
   const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
   const bool was_mod = test_bit(a.curr_mod, v);
@@ -399,14 +393,14 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int
   uint32_t prev[D], cur[D];
   bool loaded = false;
   uint32_t nvisit = 0;
-  if (check) merge_successors<R, G, SPL>(a, v, gl, beg, end, G, prev, cur, loaded, nvisit);
+  if (check) merge_successors<R, G, SPL>(a, v, gl, beg, end, G, prev, cur, loaded, nvisit, ids);
   finish_node<R, G, SPL>(a, v, gl, check, was_mod, loaded, prev, cur, nvisit,
                          check ? end - beg : 0, acc_out);
 }
 
 // Each wavefront takes chunks of kRoundsPerChunk * (64 / G) positions and
 // gives every lane group one node per round.
-constexpr int kRoundsPerChunk = 16;
+constexpr int kRoundsPerChunk = 8;
 
 template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
@@ -416,6 +410,8 @@ __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
   const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
   const uint64_t chunks = (count + CH - 1) / CH;
+  __shared__ uint32_t s_ids[kBlock];
+  uint32_t *ids = s_ids + (threadIdx.x / G) * G;
   Acc acc;
   for (;;) {
     uint64_t c = 0;
@@ -427,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
       const uint64_t i = c * CH + uint64_t(round) * NPW + g;
       if (i < count) {
         const uint32_t v = (a.flags & F_LIST) ? a.list[i] : (a.order ? a.order[i] : uint32_t(i));
-        process_node<R, G, SPL>(a, v, gl, acc);
+        process_node<R, G, SPL>(a, v, gl, acc, ids);
       }
     }
   }
@@ -441,6 +437,7 @@ template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
   constexpr int D = R * SPL, NG = kBlock / G;
   __shared__ uint32_t s_acc[NG][G * D];
+  __shared__ uint32_t s_ids[kBlock];
   __shared__ uint32_t s_visit[NG];
   __shared__ uint32_t s_any;
   const int g = threadIdx.x / G, gl = threadIdx.x % G;
@@ -463,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
     uint32_t nvisit = 0;
     if (check)
       merge_successors<R, G, SPL>(a, v, gl, beg + uint64_t(g) * G, end, uint64_t(NG) * G, prev,
-                                  cur, loaded, nvisit);
+                                  cur, loaded, nvisit, s_ids + g * G);
     if (threadIdx.x == 0) s_any = 0;
     __syncthreads();
     if (loaded && gl == 0) atomicOr(&s_any, 1u);
