@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(REPO, "webgraph-algo-rs_amd")
 PEAK_HBM_GBS = 8000.0
@@ -44,7 +46,6 @@ def algorithmic_bytes(s, n, counter_bytes, outputs):
 def cpu_baseline(orc, hbm, args, log2m, reg_bits):
     """The oracle's OpenMP restatement of the reference engine, on a bounded
     sample of the same workload: the same generator at n = args.cpu_nodes."""
-    import numpy as np
     g = hbm.Graph.web(args.cpu_nodes, 2.1, 3, 10000, 1000, 0.7, 2)
     gt = g.transpose()
     threads = os.cpu_count() or 1
@@ -71,6 +72,74 @@ def cpu_baseline(orc, hbm, args, log2m, reg_bits):
                       f"r={reg_bits}, {ref.iteration} iterations in {dt:.1f}s"}
 
 
+def batch_workload(hbm, args):
+    """64 independent instances: sizes log-uniform on [10^4, 2*10^6], each
+    graph from the same generator with its own seed, log2m cycling over
+    {6, 8, 10}, 6-bit registers, Jenkins seed per instance."""
+    rng = np.random.default_rng(3)
+    ns = np.exp(rng.uniform(np.log(1e4), np.log(2e6), args.batch)).astype(np.int64)
+    builders = []
+    for i, n in enumerate(ns):
+        g = hbm.Graph.web(int(n), 2.1, 3, 10000, 1000, 0.7, 1000 + i)
+        gt = g.transpose()
+        builders.append(hbm.HyperBallBuilder.with_hyper_log_log(g, gt, (6, 8, 10)[i % 3])
+                        .register_bits(args.reg_bits).jenkins(42 + i)
+                        .sum_of_distances(True).sum_of_inverse_distances(True))
+    return builders
+
+
+def bench_batch(hbm, args):
+    """Batched run (one fused launch sequence per batched iteration, replayed
+    from a HIP graph) against a sequential loop of single-instance runs of
+    the same instances; both produce identical results."""
+    t = time.perf_counter()
+    builders = batch_workload(hbm, args)
+    nodes = sum(b._graph.num_nodes for b in builders)
+    arcs = sum(b._graph.num_arcs for b in builders)
+    print(f"# batch of {len(builders)} graphs, {nodes} nodes, {arcs} arcs, built in "
+          f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+
+    def work_of(instances):
+        return sum(s["visited_arcs"] * h.params()["num_registers"]
+                   for h in instances for s in h.iteration_stats())
+
+    batch = hbm.HyperBallBatch(builders)
+    for _ in range(args.warmup):
+        batch.run_until_done()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run_until_done()
+    batched_s = (time.perf_counter() - t0) / args.steps
+    work = work_of(batch.instances)
+    steps_ms = batch.step_times()
+    nf_batch = [h.neighborhood_function() for h in batch.instances]
+    batch.close()
+    print(f"# batched: {batched_s * 1e3:.1f} ms per run, {len(steps_ms)} batched iterations",
+          file=sys.stderr, flush=True)
+
+    singles = [b.build() for b in builders]
+    for _ in range(args.warmup):
+        for h in singles:
+            h.run_until_done()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for h in singles:
+            h.run_until_done()
+    seq_s = (time.perf_counter() - t0) / args.steps
+    same = all(np.array_equal(a, h.neighborhood_function()) for a, h in zip(nf_batch, singles))
+    assert work == work_of(singles)
+    for h in singles:
+        h.close()
+    print(f"# sequential: {seq_s * 1e3:.1f} ms per run", file=sys.stderr, flush=True)
+    return {"instances": len(builders), "nodes": nodes, "arcs": arcs,
+            "log2m": [6, 8, 10], "register_bits": args.reg_bits,
+            "batched_value": work / batched_s, "batched_ms": batched_s * 1e3,
+            "batched_iterations": len(steps_ms),
+            "sequential_value": work / seq_s, "sequential_ms": seq_s * 1e3,
+            "speedup": seq_s / batched_s, "identical_results": bool(same),
+            "unit": "arc-registers/s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +151,8 @@ def main():
     ap.add_argument("--cpu-nodes", type=int, default=1_000_000)
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=64, help="instances in the batched leg (0 = skip)")
+    ap.add_argument("--only-batch", action="store_true")
     ap.add_argument("--stats", action="store_true", help="print per-iteration statistics to stderr")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size "
@@ -107,6 +178,10 @@ def main():
     hbm = _load("hyperball", os.path.join(PKG, "hyperball.py"))
     if not os.path.exists(hbm.LIB_PATH):
         _load("graft_entry", os.path.join(REPO, "__graft_entry__.py")).build()
+
+    if args.only_batch:
+        print(json.dumps({"batched": bench_batch(hbm, args)}), flush=True)
+        return
 
     t = time.perf_counter()
     g = hbm.Graph.web(args.nodes, 2.1, 3, 10000, 1000, 0.7, 2)
@@ -170,6 +245,9 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_total / max(1, launches)},
     }
     hb.close()
+    del g, gt
+    if args.batch:
+        result["batched"] = bench_batch(hbm, args)
     if not args.no_cpu_baseline:
         orc = _load("hb_oracle_py", os.path.join(REPO, "oracle", "oracle.py"))
         result["cpu_baseline"] = cpu_baseline(orc, hbm, args, args.log2m, args.reg_bits)

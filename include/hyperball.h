@@ -214,6 +214,29 @@ hb_status hb_set_timing(hb_instance *hb, int enabled);
  * this go to the workgroup-per-node kernel. */
 hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree);
 
+/* ---- batches ------------------------------------------------------------
+ * No counterpart in the reference, which runs one HyperBall at a time: many
+ * independent instances (each its own graph, configuration and stop rule)
+ * advance together, one fused launch sequence per batched iteration, replayed
+ * from a HIP graph.  Each instance keeps the single-instance semantics and
+ * accessors.  A batch holds log2m 6, 8 or 10 with 5- or 6-bit registers. */
+typedef struct hb_batch hb_batch;
+/* transposes: NULL, or k entries where offsets == NULL means "no transpose". */
+hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transposes,
+                          const hb_config *cfgs, hb_batch **out);
+hb_status hb_batch_destroy(hb_batch *b);
+/* Borrowed handle of instance i for the accessors and hb_add_discount_function;
+ * hb_destroy refuses it. */
+hb_status hb_batch_instance(hb_batch *b, size_t i, hb_instance **out);
+/* HyperBall::run semantics per instance; the batch ends when every instance
+ * has stopped. */
+hb_status hb_batch_run(hb_batch *b, uint64_t upper_bound, double threshold);
+hb_status hb_batch_run_until_done(hb_batch *b);
+/* HIP-graph replay of the batched iteration on (default) or off. */
+hb_status hb_batch_set_graph(hb_batch *b, int enabled);
+/* Device time of every batched iteration of the last run, in milliseconds. */
+hb_status hb_batch_step_times(hb_batch *b, double *out, size_t cap, size_t *len);
+
 #ifdef __cplusplus
 }
 #endif

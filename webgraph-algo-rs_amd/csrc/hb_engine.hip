@@ -76,6 +76,8 @@ const char *kNotRun =
 struct Instance {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
+  bool in_batch = false;  // owned by a batch: hb_destroy refuses it
   uint32_t cus = 256;
 
   uint64_t n = 0, a = 0;
@@ -137,7 +139,7 @@ struct Instance {
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (h_stats) (void)hipHostFree(h_stats);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 
   size_t acc_bytes() const { return acc_mode == ACC_F32 ? 4 : 8; }
@@ -191,50 +193,50 @@ struct Instance {
     inited = true;
   }
 
-  void iterate() {
+  // Host part of HyperBall::iterate before the device work: mode choice
+  // (hyperball_impl.rs:786-861) and the arguments of the union launch.
+  struct Plan {
+    UnionArgs ua;
+    bool list_mode;
+    double base_nf;
+    uint32_t grid, heavy_grid;
+  };
+
+  Plan plan_iteration() {
     if (!inited) fail(HB_E_INVALID_ARG, "hb_iterate called before hb_init");
-    use_device();
+    Plan p{};
     const uint64_t modified_prev = modified;
     const bool prev_systolic = systolic;
     systolic = has_t && iteration > 0 && modified_prev < n / 4;
-    const double base_nf = systolic ? last : 0.0;
+    p.base_nf = systolic ? last : 0.0;
     local = pre_local;
     pre_local = systolic && a > 0 && modified_prev < (n * n) / (a * 10);
 
-    uint32_t *cmod = mod[mcur].p, *nmod = mod[1 - mcur].p;
-    uint32_t *cmust = must[mustcur].p, *nmust = must[1 - mustcur].p;
-    if (timing) HB_HIP(hipEventRecord(ev[0], stream));
-    fill_zero(nmod, bitmap_words * 4);
-    if (systolic) fill_zero(nmust, bitmap_words * 4);
-
     uint32_t flags = 0;
-    bool list_mode = false;
+    p.list_mode = false;
     if (systolic) flags |= F_SYSTOLIC;
     if (do_centrality()) flags |= F_CENTRALITY;
     if (local) {
-      list_mode = true;
+      p.list_mode = true;
       flags |= F_CHECK_ALL;
     } else if (systolic && prev_systolic) {
-      list_mode = true;
+      p.list_mode = true;
     } else {
       flags |= F_CHECK_ALL;  // standard, or the first systolic iteration (all nodes checked)
     }
-    if (list_mode) {
-      HB_HIP(launch_compact(cmust, cmod, n, bsum.p, list.p, list_len.p, stream));
-      flags |= F_LIST;
-    }
+    if (p.list_mode) flags |= F_LIST;
 
-    UnionArgs ua{};
+    UnionArgs &ua = p.ua;
     ua.off = off.p;
     ua.succ = succ.p;
     ua.toff = toff.p;
     ua.tsucc = tsucc.p;
     ua.curr = cnt[cur].p;
     ua.next = cnt[1 - cur].p;
-    ua.curr_mod = cmod;
-    ua.next_mod = nmod;
-    ua.must = cmust;
-    ua.next_must = nmust;
+    ua.curr_mod = mod[mcur].p;
+    ua.next_mod = mod[1 - mcur].p;
+    ua.must = must[mustcur].p;
+    ua.next_must = must[1 - mustcur].p;
     ua.cache = cache.p;
     ua.list = list.p;
     ua.list_len = list_len.p;
@@ -255,26 +257,24 @@ struct Instance {
     ua.lc_table = lc.p;
     ua.stats = stats.p;
     ua.work = work.p;
+    ua.heavy_degree = heavy_count ? heavy_degree : UINT64_MAX;
+    ua.heavy_list = d_heavy.p;
+    ua.heavy_count = heavy_count;
 
     int lanes = 1, spl = 1;
     union_shape(int(r), int(log2m), &lanes, &spl);
     const uint64_t npb = kBlock / lanes;
-    uint32_t grid = max_grid;
-    if (!list_mode) grid = uint32_t(std::min<uint64_t>(max_grid, std::max<uint64_t>(8, (n + npb - 1) / npb)));
-    grid = (grid + 7) / 8 * 8;
-    const uint32_t heavy_grid = uint32_t(std::min<uint64_t>(heavy_count, kHeavyGridCap));
-    ua.heavy_degree = heavy_count ? heavy_degree : UINT64_MAX;
-    ua.heavy_list = d_heavy.p;
-    ua.heavy_count = heavy_count;
-    HB_HIP(launch_reset_stats(stats.p, work.p, stream));
-    if (timing) HB_HIP(hipEventRecord(ev[1], stream));
-    HB_HIP(launch_union(int(r), int(log2m), ua, grid, heavy_grid, stream));
-    if (timing) HB_HIP(hipEventRecord(ev[2], stream));
-    HB_HIP(hipMemcpyAsync(h_stats, stats.p, sizeof(Stats), hipMemcpyDeviceToHost, stream));
-    if (timing) HB_HIP(hipEventRecord(ev[3], stream));
-    HB_HIP(hipStreamSynchronize(stream));
-    const Stats st = *h_stats;
+    p.grid = max_grid;
+    if (!p.list_mode)
+      p.grid = uint32_t(std::min<uint64_t>(max_grid, std::max<uint64_t>(8, (n + npb - 1) / npb)));
+    p.grid = (p.grid + 7) / 8 * 8;
+    p.heavy_grid = uint32_t(std::min<uint64_t>(heavy_count, kHeavyGridCap));
+    return p;
+  }
 
+  // Host part of HyperBall::iterate after the device work
+  // (hyperball_impl.rs:909-946): swaps, neighbourhood function, counters.
+  void finish_iteration(const Plan &p, const Stats &st, float union_ms, float iteration_ms) {
     hb_iteration_stats hs{};
     hs.iteration = iteration;
     hs.systolic = systolic;
@@ -286,18 +286,13 @@ struct Instance {
     hs.transpose_arcs = st.tscanned;
     hs.copied = st.copied;
     hs.modified = st.modified;
-    if (timing) {
-      float ms = 0;
-      HB_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
-      hs.union_ms = ms;
-      HB_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
-      hs.iteration_ms = ms;
-    }
+    hs.union_ms = union_ms;
+    hs.iteration_ms = iteration_ms;
 
     cur = 1 - cur;
     mcur = 1 - mcur;
     if (systolic) mustcur = 1 - mustcur;
-    double current = base_nf + stats_nf(st);
+    double current = p.base_nf + stats_nf(st);
     last = current;
     const double last_output = nf.back();
     if (current < last_output) current = last_output;
@@ -309,13 +304,42 @@ struct Instance {
     iteration++;
   }
 
+  void iterate() {
+    use_device();
+    Plan p = plan_iteration();
+    if (timing) HB_HIP(hipEventRecord(ev[0], stream));
+    fill_zero(p.ua.next_mod, bitmap_words * 4);
+    if (systolic) fill_zero(p.ua.next_must, bitmap_words * 4);
+    if (p.list_mode)
+      HB_HIP(launch_compact(p.ua.must, p.ua.curr_mod, n, bsum.p, list.p, list_len.p, stream));
+    HB_HIP(launch_reset_stats(stats.p, work.p, stream));
+    if (timing) HB_HIP(hipEventRecord(ev[1], stream));
+    HB_HIP(launch_union(int(r), int(log2m), p.ua, p.grid, p.heavy_grid, stream));
+    if (timing) HB_HIP(hipEventRecord(ev[2], stream));
+    HB_HIP(hipMemcpyAsync(h_stats, stats.p, sizeof(Stats), hipMemcpyDeviceToHost, stream));
+    if (timing) HB_HIP(hipEventRecord(ev[3], stream));
+    HB_HIP(hipStreamSynchronize(stream));
+    float ums = 0, ims = 0;
+    if (timing) {
+      HB_HIP(hipEventElapsedTime(&ums, ev[1], ev[2]));
+      HB_HIP(hipEventElapsedTime(&ims, ev[0], ev[3]));
+    }
+    finish_iteration(p, *h_stats, ums, ims);
+  }
+
+  bool stop_after(uint64_t i, uint64_t upper_bound, double threshold) const {
+    if (modified == 0) return true;
+    if (!std::isnan(threshold) && i > 3 && rel_inc < 1.0 + threshold) return true;
+    return i + 1 >= upper_bound;
+  }
+
+  // HyperBall::run (hyperball_impl.rs:534-583).
   void run(uint64_t upper_bound, double threshold) {
     upper_bound = std::min(upper_bound, n);
     init();
     for (uint64_t i = 0; i < upper_bound; i++) {
       iterate();
-      if (modified == 0) break;
-      if (!std::isnan(threshold) && i > 3 && rel_inc < 1.0 + threshold) break;
+      if (stop_after(i, upper_bound, threshold)) break;
     }
   }
 
@@ -406,30 +430,150 @@ void copy_out(const std::vector<double> &v, double *out, size_t cap) {
 }
 
 }  // namespace
-}  // namespace hb
 
-using namespace hb;
+// Many independent instances advanced together: every batched iteration is
+// one fixed sequence (descriptor upload, fused kernels, statistics download)
+// that is captured once in a HIP graph and replayed.  All per-iteration
+// choices reach the device through the descriptor table, so the captured
+// kernel arguments never change.
+struct Batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<Instance>> inst;
+  DevBuf<BatchDesc> d_desc;
+  BatchDesc *h_desc = nullptr;  // pinned
+  DevBuf<Stats> d_stats;
+  Stats *h_stats = nullptr;     // pinned
+  DevBuf<uint32_t> work;
+  DevBuf<uint64_t> total_chunks;
+  uint32_t prep_blocks = 1, compact_blocks = 1, union_grid = 64, heavy_grid = 0;
+  uint64_t total_heavy = 0;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<double> step_ms;
 
-extern "C" {
+  ~Batch() {
+    if (stream) {
+      (void)hipSetDevice(device);
+      (void)hipStreamSynchronize(stream);
+    }
+    drop_graph();
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    inst.clear();
+    if (h_desc) (void)hipHostFree(h_desc);
+    if (h_stats) (void)hipHostFree(h_stats);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 
-uint32_t hb_register_size(uint64_t n) { return hb::register_size(n); }
-double hb_rel_std(uint32_t log2m) { return hb::rel_std(log2m); }
-uint32_t hb_log2_num_registers(double rsd) { return hb::log2_num_registers(rsd); }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
 
-void hb_config_init(hb_config *cfg) {
-  if (!cfg) return;
-  std::memset(cfg, 0, sizeof(*cfg));
-  cfg->log2m = 6;
-  cfg->hash = HB_HASH_SIPHASH13;
-  cfg->arc_granularity = 16 * 1024;
-  cfg->acc_precision = HB_ACC_F64;
-}
+  void refresh_heavy() {
+    uint64_t acc = 0;
+    for (size_t i = 0; i < inst.size(); i++) {
+      h_desc[i].heavy_base = acc;
+      acc += inst[i]->heavy_count;
+      h_desc[i].heavy_end = acc;
+    }
+    const uint32_t hg = uint32_t(std::min<uint64_t>(acc, Instance::kHeavyGridCap));
+    if (hg != heavy_grid || acc != total_heavy) drop_graph();
+    total_heavy = acc;
+    heavy_grid = hg;
+  }
 
-hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_config *cfg,
-                    hb_instance **out) {
-  return guard([&] {
-    if (!out) fail(HB_E_INVALID_ARG, "null output pointer");
-    *out = nullptr;
+  void enqueue_step() {
+    const int k = int(inst.size());
+    HB_HIP(hipMemcpyAsync(d_desc.p, h_desc, k * sizeof(BatchDesc), hipMemcpyHostToDevice, stream));
+    HB_HIP(launch_batch_iteration(d_desc.p, k, d_stats.p, work.p, total_chunks.p, prep_blocks,
+                                  compact_blocks, union_grid, heavy_grid, total_heavy, stream));
+    HB_HIP(hipMemcpyAsync(h_stats, d_stats.p, k * sizeof(Stats), hipMemcpyDeviceToHost, stream));
+  }
+
+  void capture() {
+    HB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_step();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HB_HIP(hipStreamEndCapture(stream, &graph));
+    HB_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+
+  void step() {
+    HB_HIP(hipEventRecord(ev[0], stream));
+    if (use_graph) {
+      if (!exec) capture();
+      HB_HIP(hipGraphLaunch(exec, stream));
+    } else {
+      enqueue_step();
+    }
+    HB_HIP(hipEventRecord(ev[1], stream));
+    HB_HIP(hipStreamSynchronize(stream));
+    float ms = 0;
+    HB_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    step_ms.push_back(ms);
+  }
+
+  // HyperBall::run (hyperball_impl.rs:534-583) for every instance; each one
+  // stops by its own rule, the batch ends when all have stopped.
+  void run(uint64_t upper_bound, double threshold) {
+    HB_HIP(hipSetDevice(device));
+    const size_t k = inst.size();
+    std::vector<uint64_t> ub(k), iters(k, 0);
+    std::vector<char> running(k, 0);
+    std::vector<Instance::Plan> plans(k);
+    for (size_t i = 0; i < k; i++) {
+      inst[i]->init();
+      ub[i] = std::min(upper_bound, inst[i]->n);
+      running[i] = ub[i] > 0;
+    }
+    refresh_heavy();
+    step_ms.clear();
+    for (;;) {
+      bool any = false;
+      for (size_t i = 0; i < k; i++) {
+        BatchDesc &d = h_desc[i];
+        if (!running[i]) {
+          d.active = 0;
+          continue;
+        }
+        any = true;
+        plans[i] = inst[i]->plan_iteration();
+        d.u = plans[i].ua;
+        d.u.stats = d_stats.p + i;
+        d.u.work = work.p;
+        d.active = 1;
+        d.list_mode = plans[i].list_mode;
+        d.clear_must = inst[i]->systolic;
+      }
+      if (!any) break;
+      step();
+      for (size_t i = 0; i < k; i++) {
+        if (!running[i]) continue;
+        inst[i]->finish_iteration(plans[i], h_stats[i], 0.0f, float(step_ms.back()));
+        if (inst[i]->stop_after(iters[i], ub[i], threshold)) running[i] = 0;
+        iters[i]++;
+      }
+    }
+  }
+};
+
+// Builds an instance (HyperBallBuilder::with_hyper_log_log + build,
+// hyperball_impl.rs:91-133, :321-403).  With shared_stream the instance works
+// on that stream instead of its own.
+std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *transpose,
+                                        const hb_config *cfg, hipStream_t shared_stream) {
     if (!cfg) fail(HB_E_INVALID_ARG, "null configuration");
     check_csr(graph, "graph");
     const uint64_t n = graph->num_nodes;
@@ -486,7 +630,12 @@ hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_confi
     I.device = cfg->device;
     I.use_device();
     I.cus = uint32_t(prop.multiProcessorCount);
-    HB_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+    if (shared_stream) {
+      I.stream = shared_stream;
+      I.own_stream = false;
+    } else {
+      HB_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+    }
     for (auto &e : I.ev) HB_HIP(hipEventCreate(&e));
     void *hs = nullptr;
     HB_HIP(hipHostMalloc(&hs, sizeof(Stats), hipHostMallocDefault));
@@ -578,12 +727,43 @@ hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_confi
     I.stats.alloc(1);
     I.work.alloc(kWorkCounters * kWorkStride);
     HB_HIP(hipStreamSynchronize(s));
-    *out = reinterpret_cast<hb_instance *>(inst.release());
+    return inst;
+}
+
+}  // namespace hb
+
+using namespace hb;
+
+extern "C" {
+
+uint32_t hb_register_size(uint64_t n) { return hb::register_size(n); }
+double hb_rel_std(uint32_t log2m) { return hb::rel_std(log2m); }
+uint32_t hb_log2_num_registers(double rsd) { return hb::log2_num_registers(rsd); }
+
+void hb_config_init(hb_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->log2m = 6;
+  cfg->hash = HB_HASH_SIPHASH13;
+  cfg->arc_granularity = 16 * 1024;
+  cfg->acc_precision = HB_ACC_F64;
+}
+
+hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_config *cfg,
+                    hb_instance **out) {
+  return guard([&] {
+    if (!out) fail(HB_E_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    *out = reinterpret_cast<hb_instance *>(make_instance(graph, transpose, cfg, nullptr).release());
   });
 }
 
 hb_status hb_destroy(hb_instance *hb) {
-  return guard([&] { delete reinterpret_cast<Instance *>(hb); });
+  return guard([&] {
+    Instance *I = reinterpret_cast<Instance *>(hb);
+    if (I && I->in_batch) fail(HB_E_INVALID_ARG, "this instance belongs to a batch; destroy the batch");
+    delete I;
+  });
 }
 
 hb_status hb_add_discount_function(hb_instance *hb, hb_discount_fn f, void *ctx) {
@@ -804,4 +984,108 @@ hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree) {
   });
 }
 
+// ---- batches ---------------------------------------------------------------
+
+hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transposes,
+                          const hb_config *cfgs, hb_batch **out) {
+  return guard([&] {
+    if (!out) fail(HB_E_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    if (k == 0 || !graphs || !cfgs) fail(HB_E_INVALID_ARG, "a batch needs k >= 1 graphs and configurations");
+    auto b = std::make_unique<Batch>();
+    b->device = cfgs[0].device;
+    for (size_t i = 0; i < k; i++) {
+      if (cfgs[i].device != b->device) fail(HB_E_INVALID_ARG, "all instances of a batch must use one device");
+      const uint32_t r = cfgs[i].reg_bits ? cfgs[i].reg_bits : 5u;
+      if (batch_shape(int(r), int(cfgs[i].log2m)) < 0)
+        fail(HB_E_INVALID_ARG, "instance " + std::to_string(i) +
+                                   ": a batch holds log2m 6, 8 or 10 with 5- or 6-bit registers");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      fail(HB_E_NO_DEVICE, "no HIP device is visible");
+    }
+    HB_HIP(hipSetDevice(b->device));
+    HB_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto &e : b->ev) HB_HIP(hipEventCreate(&e));
+    uint64_t max_words = 1, max_nodes = 1;
+    for (size_t i = 0; i < k; i++) {
+      const hb_csr *t = (transposes && transposes[i].offsets) ? &transposes[i] : nullptr;
+      auto I = make_instance(&graphs[i], t, &cfgs[i], b->stream);
+      I->in_batch = true;
+      max_words = std::max<uint64_t>(max_words, I->bitmap_words);
+      max_nodes = std::max<uint64_t>(max_nodes, I->n);
+      b->inst.push_back(std::move(I));
+    }
+    void *p = nullptr;
+    HB_HIP(hipHostMalloc(&p, k * sizeof(BatchDesc), hipHostMallocDefault));
+    b->h_desc = static_cast<BatchDesc *>(p);
+    std::memset(b->h_desc, 0, k * sizeof(BatchDesc));
+    HB_HIP(hipHostMalloc(&p, k * sizeof(Stats), hipHostMallocDefault));
+    b->h_stats = static_cast<Stats *>(p);
+    b->d_desc.alloc(k);
+    b->d_stats.alloc(k);
+    b->work.alloc(kWorkCounters * kWorkStride);
+    b->total_chunks.alloc(1);
+    for (size_t i = 0; i < k; i++) {
+      Instance &I = *b->inst[i];
+      BatchDesc &d = b->h_desc[i];
+      d.shape = uint32_t(batch_shape(int(I.r), int(I.log2m)));
+      d.words = I.bitmap_words;
+      d.n = I.n;
+      d.bsum = I.bsum.p;
+    }
+    b->prep_blocks = uint32_t(std::min<uint64_t>(64, (max_words + kBlock - 1) / kBlock));
+    b->compact_blocks = uint32_t(std::max<uint64_t>(1, ((max_nodes + 31) / 32 + kBlock - 1) / kBlock));
+    b->union_grid = b->inst[0]->max_grid;
+    *out = reinterpret_cast<hb_batch *>(b.release());
+  });
+}
+
+hb_status hb_batch_destroy(hb_batch *b) {
+  return guard([&] { delete reinterpret_cast<Batch *>(b); });
+}
+
+static Batch *checked_batch(hb_batch *b) {
+  if (!b) fail(HB_E_INVALID_ARG, "null batch");
+  return reinterpret_cast<Batch *>(b);
+}
+
+hb_status hb_batch_instance(hb_batch *b, size_t i, hb_instance **out) {
+  return guard([&] {
+    Batch &B = *checked_batch(b);
+    if (!out) fail(HB_E_INVALID_ARG, "null output pointer");
+    if (i >= B.inst.size()) fail(HB_E_INVALID_ARG, "no such instance");
+    *out = reinterpret_cast<hb_instance *>(B.inst[i].get());
+  });
+}
+
+hb_status hb_batch_run(hb_batch *b, uint64_t upper_bound, double threshold) {
+  return guard([&] { checked_batch(b)->run(upper_bound, threshold); });
+}
+
+hb_status hb_batch_run_until_done(hb_batch *b) {
+  return guard([&] { checked_batch(b)->run(UINT64_MAX, NAN); });
+}
+
+hb_status hb_batch_set_graph(hb_batch *b, int enabled) {
+  return guard([&] {
+    Batch &B = *checked_batch(b);
+    B.use_graph = enabled != 0;
+  });
+}
+
+hb_status hb_batch_step_times(hb_batch *b, double *out, size_t cap, size_t *len) {
+  return guard([&] {
+    Batch &B = *checked_batch(b);
+    if (len) *len = B.step_ms.size();
+    if (out) {
+      if (cap < B.step_ms.size()) fail(HB_E_INVALID_ARG, "output buffer too small");
+      std::memcpy(out, B.step_ms.data(), B.step_ms.size() * sizeof(double));
+    }
+  });
+}
+
 }  // extern "C"
+

@@ -22,6 +22,18 @@
 
 #include "hb_hll.hpp"
 
+// Lane group per (r, log2m): G lanes per node and SPL slabs per lane, with
+// G * SPL = m / 32.  HB_G_<log2m> / HB_SPL_<log2m> override a row at build
+// time (the tuning sweeps use this).
+#ifndef HB_G_10
+#define HB_G_10 16
+#define HB_SPL_10 2
+#endif
+#ifndef HB_G_8
+#define HB_G_8 8
+#define HB_SPL_8 1
+#endif
+
 namespace hb {
 namespace {
 
@@ -402,45 +414,111 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int
 // gives every lane group one node per round.
 constexpr int kRoundsPerChunk = 8;
 
+template <int G>
+__host__ __device__ constexpr uint64_t chunk_positions() {
+  return uint64_t(kRoundsPerChunk) * (64 / G);
+}
+
+template <int R, int G, int SPL>
+__device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, uint64_t count,
+                                          uint32_t *s_ids, Acc &acc) {
+  constexpr int NPW = 64 / G;
+  constexpr uint64_t CH = chunk_positions<G>();
+  const int g = __lane_id() / G, gl = __lane_id() % G;
+  uint32_t *ids = s_ids + (threadIdx.x / G) * G;
+#pragma unroll 1
+  for (int round = 0; round < kRoundsPerChunk; round++) {
+    const uint64_t i = chunk * CH + uint64_t(round) * NPW + g;
+    if (i < count) {
+      const uint32_t v = (a.flags & F_LIST) ? a.list[i] : (a.order ? a.order[i] : uint32_t(i));
+      process_node<R, G, SPL>(a, v, gl, acc, ids);
+    }
+  }
+}
+
 template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
-  constexpr int NPW = 64 / G;  // nodes per wavefront and round
-  constexpr int CH = kRoundsPerChunk * NPW;
-  const int g = __lane_id() / G, gl = __lane_id() % G;
   const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
-  const uint64_t chunks = (count + CH - 1) / CH;
+  const uint64_t chunks = (count + chunk_positions<G>() - 1) / chunk_positions<G>();
   __shared__ uint32_t s_ids[kBlock];
-  uint32_t *ids = s_ids + (threadIdx.x / G) * G;
   Acc acc;
   for (;;) {
     uint64_t c = 0;
     if (__lane_id() == 0) c = grab_chunk(a.work, chunks);
     c = __shfl(c, 0);
     if (c == UINT64_MAX) break;
-#pragma unroll 1
-    for (int round = 0; round < kRoundsPerChunk; round++) {
-      const uint64_t i = c * CH + uint64_t(round) * NPW + g;
-      if (i < count) {
-        const uint32_t v = (a.flags & F_LIST) ? a.list[i] : (a.order ? a.order[i] : uint32_t(i));
-        process_node<R, G, SPL>(a, v, gl, acc, ids);
-      }
-    }
+    run_chunk<R, G, SPL>(a, c, count, s_ids, acc);
   }
   flush_stats(acc, a.stats);
 }
 
-// Nodes with more than heavy_degree successors: one workgroup per node.  Its
+// Rows longer than heavy_degree: the whole workgroup works on one node.  Its
 // kBlock/G lane groups take interleaved G-wide slices of the row, keep
-// private maxima, and fold them in LDS; lane group 0 finishes the node.
+// private maxima, and fold them in LDS; lane group 0 finishes the node.  Every
+// thread of the block calls it with the same v.  s_acc holds kBlock * D words.
+template <int R, int G, int SPL>
+__device__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc, uint32_t *s_ids,
+                           uint32_t *s_visit, uint32_t *s_any, Acc &acc) {
+  constexpr int D = R * SPL, NG = kBlock / G, CD = G * D;
+  const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  const bool in_list = !(a.flags & F_LIST) || test_bit(a.must, v) || test_bit(a.curr_mod, v);
+  const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
+  const bool was_mod = test_bit(a.curr_mod, v);
+  if (!in_list || (!check && !was_mod)) return;  // uniform over the block
+  const uint64_t beg = a.off[v], end = a.off[v + 1];
+  uint32_t prev[D], cur[D];
+  bool loaded = false;
+  uint32_t nvisit = 0;
+  if (check)
+    merge_successors<R, G, SPL>(a, v, gl, beg + uint64_t(g) * G, end, uint64_t(NG) * G, prev,
+                                cur, loaded, nvisit, s_ids + g * G);
+  if (threadIdx.x == 0) *s_any = 0;
+  __syncthreads();
+  if (loaded && gl == 0) atomicOr(s_any, 1u);
+  if (gl == 0) s_visit[g] = nvisit;
+#pragma unroll
+  for (int k = 0; k < D; k++) s_acc[g * CD + gl * D + k] = loaded ? cur[k] : 0u;
+  __syncthreads();
+  for (int half = NG / 2; half > 0; half >>= 1) {
+    if (g < half) {
+      uint32_t x[D], y[D];
+#pragma unroll
+      for (int k = 0; k < D; k++) {
+        x[k] = s_acc[g * CD + gl * D + k];
+        y[k] = s_acc[(g + half) * CD + gl * D + k];
+      }
+#pragma unroll
+      for (int k = 0; k < SPL; k++) slab_max<R>(x + k * R, y + k * R);
+#pragma unroll
+      for (int k = 0; k < D; k++) s_acc[g * CD + gl * D + k] = x[k];
+    }
+    __syncthreads();
+  }
+  if (g == 0) {
+    const bool any = *s_any != 0;
+    uint32_t total_visit = 0;
+    for (int k = 0; k < NG; k++) total_visit += s_visit[k];
+    if (any) {
+      load_part<D>(prev, a.curr + uint64_t(v) * CD + gl * D);
+#pragma unroll
+      for (int k = 0; k < D; k++) cur[k] = s_acc[gl * D + k];
+      // the folded maxima never lose the node's own registers
+#pragma unroll
+      for (int k = 0; k < SPL; k++) slab_max<R>(cur + k * R, prev + k * R);
+    }
+    finish_node<R, G, SPL>(a, v, gl, check, was_mod, any, prev, cur, total_visit,
+                           check ? end - beg : 0, acc);
+  }
+  __syncthreads();
+}
+
 template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
-  constexpr int D = R * SPL, NG = kBlock / G;
-  __shared__ uint32_t s_acc[NG][G * D];
+  __shared__ uint32_t s_acc[kBlock * R * SPL];
   __shared__ uint32_t s_ids[kBlock];
-  __shared__ uint32_t s_visit[NG];
+  __shared__ uint32_t s_visit[kBlock];
   __shared__ uint32_t s_any;
-  const int g = threadIdx.x / G, gl = threadIdx.x % G;
   __shared__ uint64_t s_h;
   Acc acc;
   for (;;) {
@@ -449,56 +527,7 @@ __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
     const uint64_t h = s_h;
     __syncthreads();
     if (h >= a.heavy_count) break;
-    const uint32_t v = a.heavy_list[h];
-    const bool in_list = !(a.flags & F_LIST) || test_bit(a.must, v) || test_bit(a.curr_mod, v);
-    const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
-    const bool was_mod = test_bit(a.curr_mod, v);
-    if (!in_list || (!check && !was_mod)) continue;  // uniform over the block
-    const uint64_t beg = a.off[v], end = a.off[v + 1];
-    uint32_t prev[D], cur[D];
-    bool loaded = false;
-    uint32_t nvisit = 0;
-    if (check)
-      merge_successors<R, G, SPL>(a, v, gl, beg + uint64_t(g) * G, end, uint64_t(NG) * G, prev,
-                                  cur, loaded, nvisit, s_ids + g * G);
-    if (threadIdx.x == 0) s_any = 0;
-    __syncthreads();
-    if (loaded && gl == 0) atomicOr(&s_any, 1u);
-    if (gl == 0) s_visit[g] = nvisit;
-#pragma unroll
-    for (int k = 0; k < D; k++) s_acc[g][gl * D + k] = loaded ? cur[k] : 0u;
-    __syncthreads();
-    for (int half = NG / 2; half > 0; half >>= 1) {
-      if (g < half) {
-        uint32_t x[D], y[D];
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-          x[k] = s_acc[g][gl * D + k];
-          y[k] = s_acc[g + half][gl * D + k];
-        }
-#pragma unroll
-        for (int k = 0; k < SPL; k++) slab_max<R>(x + k * R, y + k * R);
-#pragma unroll
-        for (int k = 0; k < D; k++) s_acc[g][gl * D + k] = x[k];
-      }
-      __syncthreads();
-    }
-    if (g == 0) {
-      const bool any = s_any != 0;
-      uint32_t total_visit = 0;
-      for (int k = 0; k < NG; k++) total_visit += s_visit[k];
-      if (any) {
-        load_part<D>(prev, a.curr + uint64_t(v) * (G * D) + gl * D);
-#pragma unroll
-        for (int k = 0; k < D; k++) cur[k] = s_acc[0][gl * D + k];
-        // the folded maxima never lose the node's own registers
-#pragma unroll
-        for (int k = 0; k < SPL; k++) slab_max<R>(cur + k * R, prev + k * R);
-      }
-      finish_node<R, G, SPL>(a, v, gl, check, was_mod, any, prev, cur, total_visit,
-                             check ? end - beg : 0, acc);
-    }
-    __syncthreads();
+    heavy_node<R, G, SPL>(a, a.heavy_list[h], s_acc, s_ids, s_visit, &s_any, acc);
   }
   flush_stats(acc, a.stats);
 }
@@ -591,8 +620,9 @@ __global__ __launch_bounds__(kBlock) void compact_count_kernel(const uint32_t *a
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(1024) void compact_scan_kernel(uint32_t *bsum, uint32_t nb,
-                                                            uint32_t *len) {
+// Exclusive scan of bsum[0, nb) in place by one 1024-thread block; *len gets
+// the total.
+__device__ void scan_block_sums(uint32_t *bsum, uint32_t nb, uint32_t *len) {
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_carry;
   if (threadIdx.x == 0) s_carry = 0;
@@ -620,6 +650,11 @@ __global__ __launch_bounds__(1024) void compact_scan_kernel(uint32_t *bsum, uint
     __syncthreads();
   }
   if (threadIdx.x == 0) *len = s_carry;
+}
+
+__global__ __launch_bounds__(1024) void compact_scan_kernel(uint32_t *bsum, uint32_t nb,
+                                                            uint32_t *len) {
+  scan_block_sums(bsum, nb, len);
 }
 
 __global__ __launch_bounds__(kBlock) void compact_emit_kernel(const uint32_t *a, const uint32_t *b,
@@ -668,6 +703,169 @@ __global__ void to_reference_kernel(const uint32_t *cnt, uint64_t node_begin, ui
   }
 }
 
+// ---- batched execution ----------------------------------------------------
+
+#define HB_BATCH_SHAPES(X)                                                              \
+  X(0, 5, 2, 1) X(1, 5, HB_G_8, HB_SPL_8) X(2, 5, HB_G_10, HB_SPL_10) X(3, 6, 2, 1)       \
+  X(4, 6, HB_G_8, HB_SPL_8) X(5, 6, HB_G_10, HB_SPL_10)
+
+__device__ __forceinline__ uint64_t shape_chunk_positions(uint32_t shape) {
+  switch (shape) {
+#define X(ID, RR, GG, SS) \
+  case ID: return chunk_positions<GG>();
+    HB_BATCH_SHAPES(X)
+#undef X
+  }
+  return 1;
+}
+
+// First instance whose [base, end) range of work items contains x.
+__device__ __forceinline__ int find_instance(const BatchDesc *d, int k, uint64_t x, bool heavy) {
+  int lo = 0, hi = k - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    const uint64_t e = heavy ? d[mid].heavy_end : d[mid].chunk_end;
+    if (e > x) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// grid (x, instance): clears the next modified bitmap (and next must-be-checked
+// bitmap in systolic iterations) of active instances, and the statistics.
+__global__ __launch_bounds__(kBlock) void batch_prep_kernel(const BatchDesc *d, Stats *stats,
+                                                            uint32_t *work) {
+  const BatchDesc &x = d[blockIdx.y];
+  if (blockIdx.x == 0) {
+    uint32_t *p = reinterpret_cast<uint32_t *>(stats + blockIdx.y);
+    for (uint32_t i = threadIdx.x; i < sizeof(Stats) / 4; i += blockDim.x) p[i] = 0;
+    if (blockIdx.y == 0)
+      for (uint32_t i = threadIdx.x; i < kWorkCounters * kWorkStride; i += blockDim.x) work[i] = 0;
+  }
+  if (!x.active) return;
+  for (uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < x.words;
+       w += uint64_t(gridDim.x) * blockDim.x) {
+    x.u.next_mod[w] = 0;
+    if (x.clear_must) x.u.next_must[w] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void batch_compact_count_kernel(const BatchDesc *d) {
+  const BatchDesc &x = d[blockIdx.y];
+  if (!x.active || !x.list_mode) return;
+  const uint64_t nb = ((x.n + 31) / 32 + kBlock - 1) / kBlock;
+  if (blockIdx.x >= nb) return;
+  const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint32_t tot;
+  block_exclusive_scan(__popc(union_word(x.u.must, x.u.curr_mod, w, x.n)), &tot);
+  if (threadIdx.x == 0) x.bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void batch_compact_scan_kernel(const BatchDesc *d) {
+  const BatchDesc &x = d[blockIdx.x];
+  if (!x.active || !x.list_mode) return;
+  const uint32_t nb = uint32_t(((x.n + 31) / 32 + kBlock - 1) / kBlock);
+  scan_block_sums(x.bsum, nb, const_cast<uint32_t *>(x.u.list_len));
+}
+
+__global__ __launch_bounds__(kBlock) void batch_compact_emit_kernel(const BatchDesc *d) {
+  const BatchDesc &x = d[blockIdx.y];
+  if (!x.active || !x.list_mode) return;
+  const uint64_t nb = ((x.n + 31) / 32 + kBlock - 1) / kBlock;
+  if (blockIdx.x >= nb) return;
+  const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  uint32_t bits = union_word(x.u.must, x.u.curr_mod, w, x.n);
+  uint32_t tot;
+  uint32_t pos = x.bsum[blockIdx.x] + block_exclusive_scan(__popc(bits), &tot);
+  uint32_t *list = const_cast<uint32_t *>(x.u.list);
+  while (bits) {
+    const int b = __builtin_ctz(bits);
+    bits &= bits - 1;
+    list[pos++] = uint32_t(w * 32 + b);
+  }
+}
+
+// One thread: positions of every active instance, chunk prefix sums.
+__global__ void batch_plan_kernel(BatchDesc *d, int k, Stats *stats, uint64_t *total_chunks) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint64_t acc = 0;
+  for (int i = 0; i < k; i++) {
+    uint64_t count = 0;
+    if (d[i].active) count = d[i].list_mode ? uint64_t(*d[i].u.list_len) : d[i].n;
+    stats[i].list_len = count;
+    const uint64_t ch = shape_chunk_positions(d[i].shape);
+    d[i].chunk_base = acc;
+    acc += (count + ch - 1) / ch;
+    d[i].chunk_end = acc;
+  }
+  *total_chunks = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d, int k,
+                                                             const uint64_t *total_chunks,
+                                                             uint32_t *work) {
+  __shared__ uint32_t s_ids[kBlock];
+  const uint64_t chunks = *total_chunks;
+  Acc acc;
+  int current = -1;
+  for (;;) {
+    uint64_t c = 0;
+    if (__lane_id() == 0) c = grab_chunk(work, chunks);
+    c = __shfl(c, 0);
+    if (c == UINT64_MAX) break;
+    const int i = find_instance(d, k, c, false);
+    if (i != current) {
+      if (current >= 0) flush_stats(acc, d[current].u.stats);
+      acc = Acc();
+      current = i;
+    }
+    const BatchDesc &x = d[i];
+    const uint64_t count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
+    switch (x.shape) {
+#define X(ID, RR, GG, SS) \
+  case ID: run_chunk<RR, GG, SS>(x.u, c - x.chunk_base, count, s_ids, acc); break;
+      HB_BATCH_SHAPES(X)
+#undef X
+    }
+  }
+  if (current >= 0) flush_stats(acc, d[current].u.stats);
+}
+
+__global__ __launch_bounds__(kBlock) void batch_heavy_kernel(const BatchDesc *d, int k,
+                                                             uint64_t total_heavy,
+                                                             uint32_t *work) {
+  __shared__ uint32_t s_acc[kBlock * 12];
+  __shared__ uint32_t s_ids[kBlock];
+  __shared__ uint32_t s_visit[kBlock];
+  __shared__ uint32_t s_any;
+  __shared__ uint64_t s_h;
+  Acc acc;
+  int current = -1;
+  for (;;) {
+    if (threadIdx.x == 0) s_h = atomicAdd(work + 8 * kWorkStride, 1u);
+    __syncthreads();
+    const uint64_t h = s_h;
+    __syncthreads();
+    if (h >= total_heavy) break;
+    const int i = find_instance(d, k, h, true);
+    const BatchDesc &x = d[i];
+    if (!x.active) continue;
+    if (i != current) {
+      if (current >= 0) flush_stats(acc, d[current].u.stats);
+      acc = Acc();
+      current = i;
+    }
+    const uint32_t v = x.u.heavy_list[h - x.heavy_base];
+    switch (x.shape) {
+#define X(ID, RR, GG, SS) \
+  case ID: heavy_node<RR, GG, SS>(x.u, v, s_acc, s_ids, s_visit, &s_any, acc); break;
+      HB_BATCH_SHAPES(X)
+#undef X
+    }
+  }
+  if (current >= 0) flush_stats(acc, d[current].u.stats);
+}
+
 uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
   uint64_t g = (items + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -691,17 +889,6 @@ hipError_t launch_count_t(const uint32_t *c, uint64_t n, double amm, uint32_t m,
 
 }  // namespace
 
-// Lane group per (r, log2m): G lanes per node and SPL slabs per lane, with
-// G * SPL = m / 32.  HB_G_<log2m> / HB_SPL_<log2m> override a row at build
-// time (the tuning sweeps use this).
-#ifndef HB_G_10
-#define HB_G_10 16
-#define HB_SPL_10 2
-#endif
-#ifndef HB_G_8
-#define HB_G_8 8
-#define HB_SPL_8 1
-#endif
 #define HB_SHAPES(X)                                                                           \
   X(5, 5, 1, 1) X(5, 6, 2, 1) X(5, 7, 4, 1) X(5, 8, HB_G_8, HB_SPL_8) X(5, 9, 8, 2)             \
   X(5, 10, HB_G_10, HB_SPL_10) X(5, 11, 32, 2) X(5, 12, 64, 2) X(6, 5, 1, 1) X(6, 6, 2, 1)       \
@@ -727,6 +914,25 @@ hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, uin
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
+}
+
+int batch_shape(int r, int log2m) {
+  if ((r != 5 && r != 6) || (log2m != 6 && log2m != 8 && log2m != 10)) return -1;
+  return (r - 5) * 3 + (log2m - 6) / 2;
+}
+
+hipError_t launch_batch_iteration(BatchDesc *d, int k, Stats *stats, uint32_t *work,
+                                  uint64_t *total_chunks, uint32_t prep_blocks,
+                                  uint32_t compact_blocks, uint32_t union_grid,
+                                  uint32_t heavy_grid, uint64_t total_heavy, hipStream_t s) {
+  batch_prep_kernel<<<dim3(prep_blocks, k), kBlock, 0, s>>>(d, stats, work);
+  batch_compact_count_kernel<<<dim3(compact_blocks, k), kBlock, 0, s>>>(d);
+  batch_compact_scan_kernel<<<k, 1024, 0, s>>>(d);
+  batch_compact_emit_kernel<<<dim3(compact_blocks, k), kBlock, 0, s>>>(d);
+  batch_plan_kernel<<<1, 64, 0, s>>>(d, k, stats, total_chunks);
+  if (heavy_grid) batch_heavy_kernel<<<heavy_grid, kBlock, 0, s>>>(d, k, total_heavy, work);
+  batch_union_kernel<<<union_grid, kBlock, 0, s>>>(d, k, total_chunks, work);
+  return hipGetLastError();
 }
 
 hipError_t launch_count_all(int r, int log2m, const uint32_t *c, uint64_t n, double amm, uint32_t m,

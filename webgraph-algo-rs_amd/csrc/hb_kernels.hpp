@@ -101,6 +101,32 @@ struct SeedArgs {
   uint64_t weight_seed;
 };
 
+// ---- batched execution --------------------------------------------------
+// A batch runs many independent instances through one launch per step.  Each
+// instance is described by a BatchDesc; kernels find the instance of a work
+// item from prefix sums in the table and branch on its shape (uniform per
+// wavefront).  Shapes in a batch: (r, log2m) in {5, 6} x {6, 8, 10}.
+constexpr int kBatchShapes = 6;
+int batch_shape(int r, int log2m);  // index, or -1 when a batch cannot hold it
+
+struct BatchDesc {
+  UnionArgs u;
+  uint32_t shape, active, list_mode, clear_must;
+  uint64_t words;  // 32-bit words of each bitmap
+  uint64_t n;
+  uint32_t *bsum;  // compaction scratch
+  uint64_t chunk_base, chunk_end;  // written on the device by the plan step
+  uint64_t heavy_base, heavy_end;  // prefix of the instances' heavy rows
+};
+
+// Launches one batched iteration (all instances) on s: clear the next
+// bitmaps and statistics, compact the worklists of list-mode instances, plan
+// the chunk space, then the heavy-row and union kernels.
+hipError_t launch_batch_iteration(BatchDesc *d, int k, Stats *stats, uint32_t *work,
+                                  uint64_t *total_chunks, uint32_t prep_blocks,
+                                  uint32_t compact_blocks, uint32_t union_grid,
+                                  uint32_t heavy_grid, uint64_t total_heavy, hipStream_t s);
+
 // Chooses the lane group of a (r, log2m) shape: lanes per node and slabs per
 // lane.  Returns false for shapes without a kernel.
 bool union_shape(int r, int log2m, int *lanes, int *slabs_per_lane);

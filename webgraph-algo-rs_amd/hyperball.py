@@ -121,6 +121,13 @@ _SIGNATURES = [
     ("hb_get_iteration_stats", _I, [_P, _U64, _P]),
     ("hb_set_timing", _I, [_P, _I]),
     ("hb_set_tuning", _I, [_P, _U32]),
+    ("hb_batch_create", _I, [_SZ, _P, _P, _P, _P]),
+    ("hb_batch_destroy", _I, [_P]),
+    ("hb_batch_instance", _I, [_P, _SZ, _P]),
+    ("hb_batch_run", _I, [_P, _U64, _D]),
+    ("hb_batch_run_until_done", _I, [_P]),
+    ("hb_batch_set_graph", _I, [_P, _I]),
+    ("hb_batch_step_times", _I, [_P, _P, _SZ, _P]),
 ]
 
 _lib = None
@@ -349,8 +356,7 @@ class HyperBallBuilder:
         self._device = int(d)
         return self
 
-    def build(self) -> "HyperBall":
-        """HyperBallBuilder::build (hyperball_impl.rs:321-403)."""
+    def _config(self) -> _Config:
         cfg = _Config()
         lib().hb_config_init(ctypes.byref(cfg))
         cfg.log2m = self._log2m
@@ -365,6 +371,11 @@ class HyperBallBuilder:
         cfg.sum_of_inverse_distances = int(self._hc)
         cfg.acc_precision = self._acc
         cfg.device = self._device
+        return cfg
+
+    def build(self) -> "HyperBall":
+        """HyperBallBuilder::build (hyperball_impl.rs:321-403)."""
+        cfg = self._config()
         g = self._graph.csr()
         t = self._transpose.csr() if self._transpose is not None else None
         h = ctypes.c_void_p()
@@ -379,11 +390,13 @@ class HyperBallBuilder:
 class HyperBall:
     """HyperBall (hyperball_impl.rs:477-509) backed by a device instance."""
 
-    def __init__(self, handle: ctypes.c_void_p, n: int):
+    def __init__(self, handle: ctypes.c_void_p, n: int, owner=None):
         self._h = handle
         self._n = n
         self._callbacks = []
-        self._fin = weakref.finalize(self, lib().hb_destroy, handle)
+        self._owner = owner  # a batch that owns the instance
+        self._fin = weakref.finalize(self, lambda: None) if owner is not None else \
+            weakref.finalize(self, lib().hb_destroy, handle)
 
     def close(self) -> None:
         self._fin()
@@ -484,3 +497,48 @@ class HyperBall:
         """Rows longer than heavy_degree go to the workgroup-per-node kernel
         (0 = default, 2**32 - 1 = never)."""
         _check(lib().hb_set_tuning(self._h, int(heavy_degree)))
+
+
+class HyperBallBatch:
+    """Many independent HyperBall instances advanced together (one fused
+    launch sequence per batched iteration, replayed from a HIP graph)."""
+
+    def __init__(self, builders: list):
+        k = len(builders)
+        self._builders = builders
+        self._cfgs = (_Config * k)(*[b._config() for b in builders])
+        self._gs = (_Csr * k)(*[b._graph.csr() for b in builders])
+        self._ts = (_Csr * k)(*[b._transpose.csr() if b._transpose is not None else _Csr() for b in builders])
+        h = ctypes.c_void_p()
+        _check(lib().hb_batch_create(k, self._gs, self._ts, self._cfgs, ctypes.byref(h)))
+        self._h = h
+        self._fin = weakref.finalize(self, lib().hb_batch_destroy, h)
+        self.instances = []
+        for i, b in enumerate(builders):
+            ih = ctypes.c_void_p()
+            _check(lib().hb_batch_instance(h, i, ctypes.byref(ih)))
+            hb = HyperBall(ih, b._graph.num_nodes, owner=self)
+            for f in b._discounts:
+                hb._add_discount(f)
+            self.instances.append(hb)
+
+    def close(self) -> None:
+        self.instances = []
+        self._fin()
+
+    def run(self, upper_bound: int, threshold: Optional[float] = None) -> None:
+        _check(lib().hb_batch_run(self._h, min(int(upper_bound), 2**64 - 1),
+                                  math.nan if threshold is None else float(threshold)))
+
+    def run_until_done(self) -> None:
+        _check(lib().hb_batch_run_until_done(self._h))
+
+    def set_graph(self, on: bool) -> None:
+        _check(lib().hb_batch_set_graph(self._h, int(on)))
+
+    def step_times(self) -> np.ndarray:
+        ln = ctypes.c_size_t()
+        _check(lib().hb_batch_step_times(self._h, None, 0, ctypes.byref(ln)))
+        out = np.zeros(ln.value)
+        _check(lib().hb_batch_step_times(self._h, _ptr(out), len(out), ctypes.byref(ln)))
+        return out
