@@ -272,14 +272,29 @@ __device__ __forceinline__ uint64_t grab_chunk(uint32_t *work, uint64_t chunks) 
   return UINT64_MAX;
 }
 
+// Successor ids are read in slices of slice_width<G>() consecutive positions:
+// each lane of a G-lane group reads ids_per_lane<G>() of them, so narrow
+// groups (small counters) still take 16 ids per ballot round.
+#ifndef HB_ID_SLICE
+#define HB_ID_SLICE 16
+#endif
+template <int G>
+__host__ __device__ constexpr int ids_per_lane() {
+  return G >= HB_ID_SLICE ? 1 : HB_ID_SLICE / G;
+}
+template <int G>
+__host__ __device__ constexpr int slice_width() {
+  return G * ids_per_lane<G>();
+}
+
 // Merges into `cur` the counters of the modified successors (other than v)
-// found at positions first, first + stride, ... of v's row, G ids at a time
+// found in the slices of v's row that start at first, first + stride, ...
 // (hyperball_impl.rs:1037-1049).  `prev` / `cur` are loaded from the node's
-// own counter on the first hit.  The ids of the modified successors of each
-// G-wide slice are packed, in ballot order, into the lane group's LDS slot
-// `ids` (G words); they are then merged U at a time with no per-slot
-// predication: a short last batch repeats its first id, and merging a counter
-// twice changes nothing.
+// own counter on the first hit.  The ids of the modified successors of a
+// slice are packed, in ballot order, into the lane group's LDS slot `ids`
+// (slice_width<G>() words); they are then merged U at a time with no
+// per-slot predication: a short last batch repeats its first id, and merging
+// a counter twice changes nothing.
 template <int R, int G, int SPL>
 __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v, int gl,
                                                  uint64_t first, uint64_t end, uint64_t stride,
@@ -287,20 +302,31 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
                                                  uint32_t &nvisit, uint32_t *ids) {
   constexpr int D = R * SPL;         // dwords per lane
   constexpr int CD = G * D;          // dwords per counter
+  constexpr int K = ids_per_lane<G>();
 #ifdef HB_LOADS_IN_FLIGHT
   constexpr int U = HB_LOADS_IN_FLIGHT;
 #else
-  constexpr int U = D <= 12 ? 4 : 2; // successor loads in flight per lane
+  constexpr int U = D <= 6 ? 8 : D <= 12 ? 4 : 2;  // successor loads in flight per lane
 #endif
   const uint32_t *own = a.curr + uint64_t(v) * CD + gl * D;
   const uint32_t *lane_base = a.curr + gl * D;
+  const uint64_t below = (uint64_t(1) << gl) - 1;
   for (uint64_t base = first; base < end; base += stride) {
-    const uint64_t idx = base + gl;
-    const uint32_t s = idx < end ? a.succ[idx] : v;
-    const bool f = (s != v) && test_bit(a.curr_mod, s);
-    const uint64_t mask = group_ballot<G>(f);
-    if (!mask) continue;
-    if (f) ids[__popcll(mask & ((uint64_t(1) << gl) - 1))] = s;
+    uint32_t s[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const uint64_t idx = base + uint64_t(j) * G + gl;
+      s[j] = idx < end ? a.succ[idx] : v;
+    }
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const bool f = (s[j] != v) && test_bit(a.curr_mod, s[j]);
+      const uint64_t mask = group_ballot<G>(f);
+      if (f) ids[cnt + __popcll(mask & below)] = s[j];
+      cnt += __popcll(mask);
+    }
+    if (!cnt) continue;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -310,7 +336,6 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
       for (int k = 0; k < D; k++) cur[k] = prev[k];
       loaded = true;
     }
-    const uint32_t cnt = __popcll(mask);
     nvisit += cnt;
     for (uint32_t k = 0; k < cnt; k += U) {
       uint32_t y[U][D];
@@ -405,7 +430,9 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int
   uint32_t prev[D], cur[D];
   bool loaded = false;
   uint32_t nvisit = 0;
-  if (check) merge_successors<R, G, SPL>(a, v, gl, beg, end, G, prev, cur, loaded, nvisit, ids);
+  if (check)
+    merge_successors<R, G, SPL>(a, v, gl, beg, end, slice_width<G>(), prev, cur, loaded, nvisit,
+                                ids);
   finish_node<R, G, SPL>(a, v, gl, check, was_mod, loaded, prev, cur, nvisit,
                          check ? end - beg : 0, acc_out);
 }
@@ -419,13 +446,17 @@ __host__ __device__ constexpr uint64_t chunk_positions() {
   return uint64_t(kRoundsPerChunk) * (64 / G);
 }
 
+// `wave_ids` is the calling wavefront's own LDS region, at least
+// 64 * ids_per_lane<G>() words; its lane groups take consecutive slots.  The
+// region is per wavefront, not per shape: in the batched kernel the
+// wavefronts of one workgroup may run different shapes at the same time.
 template <int R, int G, int SPL>
 __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, uint64_t count,
-                                          uint32_t *s_ids, Acc &acc) {
+                                          uint32_t *wave_ids, Acc &acc) {
   constexpr int NPW = 64 / G;
   constexpr uint64_t CH = chunk_positions<G>();
   const int g = __lane_id() / G, gl = __lane_id() % G;
-  uint32_t *ids = s_ids + (threadIdx.x / G) * G;
+  uint32_t *ids = wave_ids + g * slice_width<G>();
 #pragma unroll 1
   for (int round = 0; round < kRoundsPerChunk; round++) {
     const uint64_t i = chunk * CH + uint64_t(round) * NPW + g;
@@ -441,14 +472,16 @@ __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
   const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
   const uint64_t chunks = (count + chunk_positions<G>() - 1) / chunk_positions<G>();
-  __shared__ uint32_t s_ids[kBlock];
+  constexpr int kWaveIds = 64 * ids_per_lane<G>();
+  __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
+  uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
   Acc acc;
   for (;;) {
     uint64_t c = 0;
     if (__lane_id() == 0) c = grab_chunk(a.work, chunks);
     c = __shfl(c, 0);
     if (c == UINT64_MAX) break;
-    run_chunk<R, G, SPL>(a, c, count, s_ids, acc);
+    run_chunk<R, G, SPL>(a, c, count, wave_ids, acc);
   }
   flush_stats(acc, a.stats);
 }
@@ -471,8 +504,9 @@ __device__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc, uint
   bool loaded = false;
   uint32_t nvisit = 0;
   if (check)
-    merge_successors<R, G, SPL>(a, v, gl, beg + uint64_t(g) * G, end, uint64_t(NG) * G, prev,
-                                cur, loaded, nvisit, s_ids + g * G);
+    merge_successors<R, G, SPL>(a, v, gl, beg + uint64_t(g) * slice_width<G>(), end,
+                                uint64_t(NG) * slice_width<G>(), prev, cur, loaded, nvisit,
+                                s_ids + g * slice_width<G>());
   if (threadIdx.x == 0) *s_any = 0;
   __syncthreads();
   if (loaded && gl == 0) atomicOr(s_any, 1u);
@@ -516,7 +550,7 @@ __device__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc, uint
 template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
   __shared__ uint32_t s_acc[kBlock * R * SPL];
-  __shared__ uint32_t s_ids[kBlock];
+  __shared__ uint32_t s_ids[kBlock * ids_per_lane<G>()];
   __shared__ uint32_t s_visit[kBlock];
   __shared__ uint32_t s_any;
   __shared__ uint64_t s_h;
@@ -804,7 +838,10 @@ __global__ void batch_plan_kernel(BatchDesc *d, int k, Stats *stats, uint64_t *t
 __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d, int k,
                                                              const uint64_t *total_chunks,
                                                              uint32_t *work) {
-  __shared__ uint32_t s_ids[kBlock];
+  // per-wavefront regions sized for the narrowest batch shape (G = 2)
+  constexpr int kWaveIds = 64 * ids_per_lane<2>();
+  __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
+  uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
   const uint64_t chunks = *total_chunks;
   Acc acc;
   int current = -1;
@@ -823,7 +860,7 @@ __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d,
     const uint64_t count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
     switch (x.shape) {
 #define X(ID, RR, GG, SS) \
-  case ID: run_chunk<RR, GG, SS>(x.u, c - x.chunk_base, count, s_ids, acc); break;
+  case ID: run_chunk<RR, GG, SS>(x.u, c - x.chunk_base, count, wave_ids, acc); break;
       HB_BATCH_SHAPES(X)
 #undef X
     }
@@ -835,8 +872,8 @@ __global__ __launch_bounds__(kBlock) void batch_heavy_kernel(const BatchDesc *d,
                                                              uint64_t total_heavy,
                                                              uint32_t *work) {
   __shared__ uint32_t s_acc[kBlock * 12];
-  __shared__ uint32_t s_ids[kBlock];
-  __shared__ uint32_t s_visit[kBlock];
+  __shared__ uint32_t s_ids[kBlock * ids_per_lane<2>()];  // the narrowest batch shape
+  __shared__ uint32_t s_visit[kBlock];  // (one shape per workgroup at a time)
   __shared__ uint32_t s_any;
   __shared__ uint64_t s_h;
   Acc acc;
