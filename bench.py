@@ -72,6 +72,24 @@ def cpu_baseline(orc, hbm, args, log2m, reg_bits):
                       f"r={reg_bits}, {ref.iteration} iterations in {dt:.1f}s"}
 
 
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "union_step_web1e7_log2m10_r6_pmc.md")
+
+
+def profiled_traffic(args):
+    """HBM bytes per union step from the committed FETCH_SIZE / WRITE_SIZE
+    passes (copy-calibrated), for the configuration they were taken on."""
+    if (args.nodes, args.log2m, args.reg_bits) != (10_000_000, 10, 6):
+        return None
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_union_step:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def batch_workload(hbm, args):
     """64 independent instances: sizes log-uniform on [10^4, 2*10^6], each
     graph from the same generator with its own seed, log2m cycling over
@@ -239,8 +257,9 @@ def main():
                    "register_bits": args.reg_bits, "hash": "jenkins(42)", "iterations": iters,
                    "outputs": ["neighborhood_function", "sum_of_distances", "harmonic_centrality"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "union_kernel", "launches": launches,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": profiled_traffic(args),
+                     "kernel": "union step (heavy_kernel + union_kernel, one launch each per "
+                               "iteration)", "launches": launches,
                      "avg_launch_ms": union_ms / max(1, launches),
                      "algorithmic_bytes_per_launch": bytes_total / max(1, launches)},
     }

@@ -33,6 +33,10 @@
 #define HB_G_8 8
 #define HB_SPL_8 1
 #endif
+#ifndef HB_G_6
+#define HB_G_6 2
+#define HB_SPL_6 1
+#endif
 
 namespace hb {
 namespace {
@@ -740,8 +744,27 @@ __global__ void to_reference_kernel(const uint32_t *cnt, uint64_t node_begin, ui
 // ---- batched execution ----------------------------------------------------
 
 #define HB_BATCH_SHAPES(X)                                                              \
-  X(0, 5, 2, 1) X(1, 5, HB_G_8, HB_SPL_8) X(2, 5, HB_G_10, HB_SPL_10) X(3, 6, 2, 1)       \
-  X(4, 6, HB_G_8, HB_SPL_8) X(5, 6, HB_G_10, HB_SPL_10)
+  X(0, 5, HB_G_6, HB_SPL_6) X(1, 5, HB_G_8, HB_SPL_8) X(2, 5, HB_G_10, HB_SPL_10)             \
+  X(3, 6, HB_G_6, HB_SPL_6) X(4, 6, HB_G_8, HB_SPL_8) X(5, 6, HB_G_10, HB_SPL_10)
+
+// LDS needs of the batched kernels: the widest per-lane counter part (the
+// heavy kernel's fold buffer holds kBlock of them) and the largest per-wave
+// successor-id region, over all batched shapes.
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int batch_max_d() {
+  int d = 0;
+#define X(ID, RR, GG, SS) d = cmax(d, RR * SS);
+  HB_BATCH_SHAPES(X)
+#undef X
+  return d;
+}
+constexpr int batch_wave_ids() {
+  int w = 0;
+#define X(ID, RR, GG, SS) w = cmax(w, 64 * ids_per_lane<GG>());
+  HB_BATCH_SHAPES(X)
+#undef X
+  return w;
+}
 
 __device__ __forceinline__ uint64_t shape_chunk_positions(uint32_t shape) {
   switch (shape) {
@@ -838,8 +861,7 @@ __global__ void batch_plan_kernel(BatchDesc *d, int k, Stats *stats, uint64_t *t
 __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d, int k,
                                                              const uint64_t *total_chunks,
                                                              uint32_t *work) {
-  // per-wavefront regions sized for the narrowest batch shape (G = 2)
-  constexpr int kWaveIds = 64 * ids_per_lane<2>();
+  constexpr int kWaveIds = batch_wave_ids();
   __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
   uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
   const uint64_t chunks = *total_chunks;
@@ -871,8 +893,8 @@ __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d,
 __global__ __launch_bounds__(kBlock) void batch_heavy_kernel(const BatchDesc *d, int k,
                                                              uint64_t total_heavy,
                                                              uint32_t *work) {
-  __shared__ uint32_t s_acc[kBlock * 12];
-  __shared__ uint32_t s_ids[kBlock * ids_per_lane<2>()];  // the narrowest batch shape
+  __shared__ uint32_t s_acc[kBlock * batch_max_d()];
+  __shared__ uint32_t s_ids[(kBlock / 64) * batch_wave_ids()];
   __shared__ uint32_t s_visit[kBlock];  // (one shape per workgroup at a time)
   __shared__ uint32_t s_any;
   __shared__ uint64_t s_h;
@@ -927,9 +949,10 @@ hipError_t launch_count_t(const uint32_t *c, uint64_t n, double amm, uint32_t m,
 }  // namespace
 
 #define HB_SHAPES(X)                                                                           \
-  X(5, 5, 1, 1) X(5, 6, 2, 1) X(5, 7, 4, 1) X(5, 8, HB_G_8, HB_SPL_8) X(5, 9, 8, 2)             \
-  X(5, 10, HB_G_10, HB_SPL_10) X(5, 11, 32, 2) X(5, 12, 64, 2) X(6, 5, 1, 1) X(6, 6, 2, 1)       \
-  X(6, 7, 4, 1) X(6, 8, HB_G_8, HB_SPL_8) X(6, 9, 8, 2) X(6, 10, HB_G_10, HB_SPL_10)             \
+  X(5, 5, 1, 1) X(5, 6, HB_G_6, HB_SPL_6) X(5, 7, 4, 1) X(5, 8, HB_G_8, HB_SPL_8)              \
+  X(5, 9, 8, 2) X(5, 10, HB_G_10, HB_SPL_10) X(5, 11, 32, 2) X(5, 12, 64, 2) X(6, 5, 1, 1)        \
+  X(6, 6, HB_G_6, HB_SPL_6) X(6, 7, 4, 1) X(6, 8, HB_G_8, HB_SPL_8) X(6, 9, 8, 2)               \
+  X(6, 10, HB_G_10, HB_SPL_10)                                                                 \
   X(6, 11, 32, 2) X(6, 12, 64, 2)
 
 bool union_shape(int r, int log2m, int *lanes, int *spl) {
