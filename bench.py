@@ -158,6 +158,94 @@ def bench_batch(hbm, args):
             "unit": "arc-registers/s"}
 
 
+def rel_err(got, ref):
+    """Max and median relative error of got against ref (absolute where ref is 0)."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    err = np.abs(got - ref)
+    nz = ref != 0
+    err[nz] /= np.abs(ref[nz])
+    return {"max": float(err.max()), "median": float(np.median(err)),
+            "argmax": int(err.argmax())}
+
+
+def bench_precision(hbm, args):
+    """Accumulator precision on the web graph (n = args.nodes) with log2m 12:
+    the same run with f64, f32 Kahan-compensated and plain f32 per-node
+    accumulators.  Registers are integers in every mode; reachable counts
+    come from the integer register sums, so they are compared too."""
+    t = time.perf_counter()
+    g = hbm.Graph.web(args.nodes, 2.1, 3, 10000, 1000, 0.7, 2)
+    gt = g.transpose()
+    print(f"# graph n={g.num_nodes} arcs={g.num_arcs} built in {time.perf_counter() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    out = {}
+    ref = None
+    for name, mode in (("f64", hbm.ACC_F64), ("f32_kahan", hbm.ACC_F32_KAHAN), ("f32", hbm.ACC_F32)):
+        hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 12).register_bits(5).jenkins(42)
+              .sum_of_distances(True).sum_of_inverse_distances(True).accumulator_precision(mode)
+              .build())
+        t0 = time.perf_counter()
+        hb.run_until_done()
+        dt = time.perf_counter() - t0
+        res = {"sum_of_distances": hb.sum_of_distances(),
+               "harmonic_centrality": hb.harmonic_centralities(),
+               "reachable_nodes": hb.reachable_nodes(),
+               "neighborhood_function": hb.neighborhood_function()}
+        iters = len(hb.iteration_stats())
+        hb.close()
+        print(f"# {name}: {iters} iterations in {dt * 1e3:.0f} ms", file=sys.stderr, flush=True)
+        if ref is None:
+            ref = res
+            out["f64"] = {"run_ms": dt * 1e3, "iterations": iters}
+            continue
+        out[name] = {"run_ms": dt * 1e3, "iterations": iters,
+                     **{k: rel_err(res[k], ref[k]) for k in res}}
+    return {"workload": f"web graph n={g.num_nodes} arcs={g.num_arcs}, log2m 12, r 5, "
+                        "Jenkins 42, run to stabilisation",
+            "reference": "f64 accumulators on the device (bit-exact with the oracle's "
+                         "f64 engine in the parity tests)", "modes": out}
+
+
+def bench_large(hbm, args):
+    """One large instance: n = 10^8 nodes, the same generator with out-degree
+    Zipf(2.1) on [4, 10000] and a +-10,000 window (about 2e9 arcs), log2m 6,
+    5-bit registers, systolic frontier, f32 Kahan-compensated accumulators."""
+    t = time.perf_counter()
+    g = hbm.Graph.web(args.large_nodes, 2.1, 4, 10000, 10000, 0.7, 5)
+    print(f"# graph n={g.num_nodes} arcs={g.num_arcs} built in {time.perf_counter() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    gt = g.transpose()
+    print(f"# transpose built in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 6).register_bits(5).jenkins(42)
+          .sum_of_distances(True).sum_of_inverse_distances(True)
+          .accumulator_precision(hbm.ACC_F32_KAHAN).build())
+    hb.set_timing(True)
+    print(f"# uploaded in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    nodes, arcs = g.num_nodes, g.num_arcs
+    del g, gt
+    for i in range(args.warmup):
+        hb.run_until_done()
+        print(f"# warmup {i} done", file=sys.stderr, flush=True)
+    work, union_ms, iters = 0, 0.0, 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        hb.run_until_done()
+        st = hb.iteration_stats()
+        iters = len(st)
+        work += sum(s["visited_arcs"] for s in st) * 64
+        union_ms += sum(s["union_ms"] for s in st)
+        print(f"# step {i}: {iters} iterations", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    hb.close()
+    return {"workload": f"web graph n={nodes} arcs={arcs} (+-10000 window, out-degree Zipf(2.1) "
+                        "on [4, 10000]), log2m 6, r 5, f32 Kahan accumulators",
+            "value": work / elapsed, "unit": "arc-registers/s", "ms_per_step": elapsed * 1e3 / args.steps,
+            "iterations": iters, "union_ms_per_step": union_ms / args.steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +259,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="instances in the batched leg (0 = skip)")
     ap.add_argument("--only-batch", action="store_true")
+    ap.add_argument("--precision", action="store_true",
+                    help="only the accumulator precision study (log2m 12 on the web graph)")
+    ap.add_argument("--large", action="store_true",
+                    help="only the large instance (n = --large-nodes, log2m 6, r 5)")
+    ap.add_argument("--large-nodes", type=int, default=100_000_000)
     ap.add_argument("--stats", action="store_true", help="print per-iteration statistics to stderr")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size "
@@ -199,6 +292,12 @@ def main():
 
     if args.only_batch:
         print(json.dumps({"batched": bench_batch(hbm, args)}), flush=True)
+        return
+    if args.precision:
+        print(json.dumps({"precision": bench_precision(hbm, args)}), flush=True)
+        return
+    if args.large:
+        print(json.dumps({"large": bench_large(hbm, args)}), flush=True)
         return
 
     t = time.perf_counter()
