@@ -48,12 +48,7 @@ def cpu_baseline(orc, hbm, args, log2m, reg_bits):
     sample of the same workload: the same generator at n = args.cpu_nodes."""
     g = hbm.Graph.web(args.cpu_nodes, 2.1, 3, 10000, 1000, 0.7, 2)
     gt = g.transpose()
-    threads = os.cpu_count() or 1
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
-    threads = min(threads, 16)
+    threads = host_threads()
     ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=log2m,
                               reg_bits=reg_bits, hash=orc.HASH_JENKINS, seed=42,
                               sum_of_distances=True, sum_of_inverse_distances=True, threads=threads)
@@ -122,6 +117,10 @@ def bench_batch(hbm, args):
                    for h in instances for s in h.iteration_stats())
 
     batch = hbm.HyperBallBatch(builders)
+    chosen, table = batch.autotune()
+    print("# batch autotune: " + ", ".join(f"{'none' if t == 2**32 - 1 else t}: {ms:.2f}"
+                                           for t, ms in table) + f" -> {chosen}",
+          file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         batch.run_until_done()
     t0 = time.perf_counter()
@@ -136,6 +135,8 @@ def bench_batch(hbm, args):
           file=sys.stderr, flush=True)
 
     singles = [b.build() for b in builders]
+    for h in singles:
+        h.autotune(reps=1)
     for _ in range(args.warmup):
         for h in singles:
             h.run_until_done()
@@ -155,7 +156,57 @@ def bench_batch(hbm, args):
             "batched_iterations": len(steps_ms),
             "sequential_value": work / seq_s, "sequential_ms": seq_s * 1e3,
             "speedup": seq_s / batched_s, "identical_results": bool(same),
+            "batch_heavy_row_threshold": chosen,
             "unit": "arc-registers/s"}
+
+
+def host_threads():
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return min(threads, 16)
+
+
+def small_uniform(hbm, orc, args):
+    """The uniform random graph: n = 10^5 nodes, exactly 8 distinct uniform
+    successors each (self-loops excluded), log2m 8, register width from n,
+    the default SipHash-1-3 hasher, NF + sum of distances + harmonic
+    centrality in f64.  The GPU run (inputs resident) beside the oracle's
+    OpenMP restatement of the reference engine on the same, whole workload."""
+    g = hbm.Graph.uniform(100_000, 8, seed=1)
+    gt = g.transpose()
+    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).sum_of_distances(True)
+          .sum_of_inverse_distances(True).build())
+    m = hb.params()["num_registers"]
+    hb.run_until_done()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hb.run_until_done()
+    gpu_s = (time.perf_counter() - t0) / reps
+    st = hb.iteration_stats()
+    work = sum(s["visited_arcs"] for s in st) * m
+    nf, sd = hb.neighborhood_function(), hb.sum_of_distances()
+    hb.close()
+    out = {"workload": "uniform graph n=100000, 8 successors each, log2m 8, r from n, SipHash-1-3",
+           "iterations": len(st), "gpu": {"value": work / gpu_s, "ms_per_run": gpu_s * 1e3}}
+    if orc is not None:
+        threads = host_threads()
+        ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=8,
+                                  sum_of_distances=True, sum_of_inverse_distances=True,
+                                  threads=threads)
+        t0 = time.perf_counter()
+        ref.run()
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": work / cpu_s, "ms_per_run": cpu_s * 1e3, "cores": threads,
+                               "kind": "port"}
+        out["gpu_over_cpu"] = cpu_s / gpu_s
+        out["same_sum_of_distances"] = bool(np.array_equal(sd, ref.sum_of_distances()))
+        out["nf_max_rel_diff"] = float(np.max(np.abs(nf - ref.neighborhood_function()) / nf))
+    out["unit"] = "arc-registers/s"
+    return out
 
 
 def rel_err(got, ref):
@@ -259,6 +310,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="instances in the batched leg (0 = skip)")
     ap.add_argument("--only-batch", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the uniform n=1e5 leg")
     ap.add_argument("--precision", action="store_true",
                     help="only the accumulator precision study (log2m 12 on the web graph)")
     ap.add_argument("--large", action="store_true",
@@ -307,6 +359,10 @@ def main():
           file=sys.stderr, flush=True)
     hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, args.log2m).register_bits(args.reg_bits)
           .jenkins(42).sum_of_distances(True).sum_of_inverse_distances(True).build())
+    chosen, table = hb.autotune(reps=2)
+    print("# autotune (heavy-row threshold, dense-iteration ms): " +
+          ", ".join(f"{'none' if t == 2**32 - 1 else t}: {ms:.2f}" for t, ms in table) +
+          f" -> {chosen}", file=sys.stderr, flush=True)
     hb.set_timing(True)
     p = hb.params()
     m, counter_bytes = p["num_registers"], p["words_per_counter"] * 8
@@ -354,6 +410,7 @@ def main():
                                "+-1000 window 70% / preferential attachment 30%, systolic frontier)",
                    "nodes": g.num_nodes, "arcs": g.num_arcs, "log2m": args.log2m,
                    "register_bits": args.reg_bits, "hash": "jenkins(42)", "iterations": iters,
+                   "heavy_row_threshold": chosen,
                    "outputs": ["neighborhood_function", "sum_of_distances", "harmonic_centrality"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": profiled_traffic(args),
@@ -366,9 +423,12 @@ def main():
     del g, gt
     if args.batch:
         result["batched"] = bench_batch(hbm, args)
+    orc = None
     if not args.no_cpu_baseline:
         orc = _load("hb_oracle_py", os.path.join(REPO, "oracle", "oracle.py"))
         result["cpu_baseline"] = cpu_baseline(orc, hbm, args, args.log2m, args.reg_bits)
+    if not args.no_small:
+        result["uniform_1e5"] = small_uniform(hbm, orc, args)
     print(json.dumps(result), flush=True)
 
 

@@ -213,6 +213,15 @@ hb_status hb_set_timing(hb_instance *hb, int enabled);
  * default); for the autotuner.  heavy_degree: rows with more successors than
  * this go to the workgroup-per-node kernel. */
 hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree);
+/* Autotuner of that split (in the role of the reference's arc_granularity
+ * tuning knob, hyperball_impl.rs:271-274, which sizes CPU work units): times
+ * the first, dense iteration under each threshold 64 * 2^k that changes the
+ * set of heavy rows, and "none" (UINT32_MAX), best of `reps`; keeps the
+ * fastest.  Writes up to `cap` (threshold, ms) pairs, their count to *len and
+ * the chosen threshold to *chosen (each pointer may be NULL).  Resets the run
+ * state as hb_init does; results never depend on the threshold. */
+hb_status hb_autotune(hb_instance *hb, uint32_t reps, uint32_t *thresholds, double *ms,
+                      size_t cap, size_t *len, uint32_t *chosen);
 
 /* ---- batches ------------------------------------------------------------
  * No counterpart in the reference, which runs one HyperBall at a time: many
@@ -232,6 +241,10 @@ hb_status hb_batch_instance(hb_batch *b, size_t i, hb_instance **out);
  * has stopped. */
 hb_status hb_batch_run(hb_batch *b, uint64_t upper_bound, double threshold);
 hb_status hb_batch_run_until_done(hb_batch *b);
+/* hb_autotune for a batch: one threshold for every instance, timed on the
+ * first batched iteration (same outputs and reset as hb_autotune). */
+hb_status hb_batch_autotune(hb_batch *b, uint32_t reps, uint32_t *thresholds, double *ms,
+                            size_t cap, size_t *len, uint32_t *chosen);
 /* HIP-graph replay of the batched iteration on (default) or off. */
 hb_status hb_batch_set_graph(hb_batch *b, int enabled);
 /* Device time of every batched iteration of the last run, in milliseconds. */

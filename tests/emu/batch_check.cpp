@@ -2,7 +2,7 @@
 // configurations (test infrastructure, built by build_emu.py): every
 // instance's registers, modified bits, neighbourhood function and
 // accumulators must be identical.  Usage:
-//   batch_check K GRAPH(0/1) SEED NMIN NMAX DMAX [HEAVY]
+//   batch_check K GRAPH(0/1) SEED NMIN NMAX DMAX [HEAVY] [AUTOTUNE(0/1)]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -49,6 +49,7 @@ int main(int argc, char **argv) {
   const double nmin = std::atof(argv[4]), nmax = std::atof(argv[5]);
   const uint32_t dmax = uint32_t(std::atoi(argv[6]));
   const uint32_t heavy = argc > 7 ? uint32_t(std::atoi(argv[7])) : 0;
+  const int tune = argc > 8 ? std::atoi(argv[8]) : 0;
 
   std::mt19937_64 rng(seed);
   std::uniform_real_distribution<double> u(std::log(nmin), std::log(nmax));
@@ -87,6 +88,12 @@ int main(int argc, char **argv) {
       check(hb_batch_instance(b, size_t(i), &h), "hb_batch_instance");
       check(hb_set_tuning(h, heavy), "hb_set_tuning");
     }
+  if (tune) {
+    uint32_t chosen = 0;
+    size_t len = 0;
+    check(hb_batch_autotune(b, 1, nullptr, nullptr, 0, &len, &chosen), "hb_batch_autotune");
+    std::printf("autotune: %zu candidates, chose %u\n", len, chosen);
+  }
   check(hb_batch_run_until_done(b), "hb_batch_run_until_done");
   std::printf("batch run done\n");
   std::fflush(stdout);

@@ -29,9 +29,11 @@ def emu_exes():
     return {os.path.basename(e): e for e in exes}
 
 
-@pytest.mark.parametrize("graph,heavy", [(1, 0), (0, 64)])
-def test_emulated_batch_matches_single_instances(emu_exes, graph, heavy):
-    args = [emu_exes["batch_check"], "6", str(graph), "7", "200", "600", "900", str(heavy)]
+@pytest.mark.parametrize("graph,heavy,tune", [(1, 0, 1), (0, 64, 0)])
+def test_emulated_batch_matches_single_instances(emu_exes, graph, heavy, tune):
+    """(graph replay, batch autotuner) and (direct launches, heavy rows above 64)."""
+    args = [emu_exes["batch_check"], "6", str(graph), "7", "200", "600", "900", str(heavy),
+            str(tune)]
     r = subprocess.run(args, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:] + r.stderr[-6000:]
     assert r.stdout.count("identical") == 6

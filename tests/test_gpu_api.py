@@ -213,3 +213,24 @@ def test_web_graph_full_size_properties(hbm, orc, reg_bits):
     hb.run_until_done()
     assert np.array_equal(hb.sum_of_distances(), sd1)
     assert np.array_equal(hb.neighborhood_function(), nf)
+
+
+def test_autotune_keeps_results(hbm, orc):
+    """hb_autotune picks one of the thresholds it timed, resets the run
+    state, and the run after it equals the oracle."""
+    g = hbm.Graph.web(30000, 2.1, 1, 3000, 300, 0.7, 4)
+    gt = g.transpose()
+    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).jenkins(3).sum_of_distances(True)
+          .build())
+    hb.run_until_done()
+    chosen, table = hb.autotune(reps=2)
+    assert len(table) >= 3 and table[-1][0] == 2**32 - 1
+    assert chosen in [t for t, _ in table]
+    assert all(ms > 0 for _, ms in table)
+    with pytest.raises(hbm.HyperBallError):
+        hb.sum_of_distances()  # run state was reset
+    hb.run_until_done()
+    ref = _ref(orc, g, gt, hash=orc.HASH_JENKINS, seed=3, sum_of_distances=True)
+    ref.run()
+    assert np.array_equal(hb.counters(0), ref.counters(0))
+    assert np.array_equal(hb.sum_of_distances(), ref.sum_of_distances())

@@ -97,3 +97,13 @@ def test_batch_rejects_unsupported_shape(hbm):
     with pytest.raises(hbm.HyperBallError) as e:
         hbm.HyperBallBatch([hbm.HyperBallBuilder.with_hyper_log_log(g, None, 7)])
     assert "log2m 6, 8 or 10" in e.value.message
+
+
+def test_batch_autotune_keeps_results(hbm):
+    builders = _builders(hbm, 4, seed0=55)
+    batch = hbm.HyperBallBatch(builders)
+    chosen, table = batch.autotune()
+    assert chosen in [t for t, _ in table] and table[-1][0] == 2**32 - 1
+    batch.run_until_done()
+    for b, hb in zip(builders, batch.instances):
+        _compare(hbm, b, hb)

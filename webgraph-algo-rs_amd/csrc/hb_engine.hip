@@ -153,6 +153,48 @@ struct Instance {
     heavy_count = k;
   }
 
+  // Light/heavy split autotuner: times the first (dense, every node checked)
+  // iteration under each candidate threshold that changes the heavy-row set
+  // and keeps the fastest.  Results do not depend on the threshold; the run
+  // state is reset as by init().  Candidates: every 64 * 2^k that leaves a
+  // different, non-empty set of heavy rows, then "no heavy rows".
+  std::vector<std::pair<uint64_t, double>> autotune(uint32_t reps) {
+    if (reps == 0) reps = 1;
+    std::vector<uint64_t> cand;
+    uint64_t last_count = UINT64_MAX;
+    for (uint64_t t = kHeavyFloor;; t *= 2) {
+      set_heavy_degree(t);
+      if (heavy_count == 0) break;
+      if (heavy_count != last_count) cand.push_back(t);
+      last_count = heavy_count;
+    }
+    cand.push_back(UINT64_MAX);
+    const bool was_timing = timing;
+    timing = true;
+    std::vector<std::pair<uint64_t, double>> table;
+    try {
+      for (uint64_t t : cand) {
+        set_heavy_degree(t);
+        double best = 1e300;
+        for (uint32_t r = 0; r < reps; r++) {
+          init();
+          iterate();
+          best = std::min(best, history.back().union_ms);
+        }
+        table.push_back({t, best});
+      }
+    } catch (...) {
+      timing = was_timing;
+      throw;
+    }
+    timing = was_timing;
+    auto it = std::min_element(table.begin(), table.end(),
+                               [](const auto &a, const auto &b) { return a.second < b.second; });
+    set_heavy_degree(it->first);
+    init();
+    return table;
+  }
+
   void fill_zero(void *p, size_t bytes) {
     HB_HIP(launch_fill_u32(static_cast<uint32_t *>(p), bytes / 4, 0u, stream));
   }
@@ -525,14 +567,20 @@ struct Batch {
     step_ms.push_back(ms);
   }
 
-  // HyperBall::run (hyperball_impl.rs:534-583) for every instance; each one
-  // stops by its own rule, the batch ends when all have stopped.
-  void run(uint64_t upper_bound, double threshold) {
+  // Per-instance run state of a batched run.
+  std::vector<uint64_t> ub, iters;
+  std::vector<char> running;
+  std::vector<Instance::Plan> plans;
+  double stop_threshold = NAN;
+
+  void begin(uint64_t upper_bound, double threshold) {
     HB_HIP(hipSetDevice(device));
     const size_t k = inst.size();
-    std::vector<uint64_t> ub(k), iters(k, 0);
-    std::vector<char> running(k, 0);
-    std::vector<Instance::Plan> plans(k);
+    ub.assign(k, 0);
+    iters.assign(k, 0);
+    running.assign(k, 0);
+    plans.resize(k);
+    stop_threshold = threshold;
     for (size_t i = 0; i < k; i++) {
       inst[i]->init();
       ub[i] = std::min(upper_bound, inst[i]->n);
@@ -540,32 +588,77 @@ struct Batch {
     }
     refresh_heavy();
     step_ms.clear();
-    for (;;) {
-      bool any = false;
-      for (size_t i = 0; i < k; i++) {
-        BatchDesc &d = h_desc[i];
-        if (!running[i]) {
-          d.active = 0;
-          continue;
-        }
-        any = true;
-        plans[i] = inst[i]->plan_iteration();
-        d.u = plans[i].ua;
-        d.u.stats = d_stats.p + i;
-        d.u.work = work.p;
-        d.active = 1;
-        d.list_mode = plans[i].list_mode;
-        d.clear_must = inst[i]->systolic;
+  }
+
+  // One batched iteration of every running instance; false when none runs.
+  bool advance() {
+    const size_t k = inst.size();
+    bool any = false;
+    for (size_t i = 0; i < k; i++) {
+      BatchDesc &d = h_desc[i];
+      if (!running[i]) {
+        d.active = 0;
+        continue;
       }
-      if (!any) break;
-      step();
-      for (size_t i = 0; i < k; i++) {
-        if (!running[i]) continue;
-        inst[i]->finish_iteration(plans[i], h_stats[i], 0.0f, float(step_ms.back()));
-        if (inst[i]->stop_after(iters[i], ub[i], threshold)) running[i] = 0;
-        iters[i]++;
-      }
+      any = true;
+      plans[i] = inst[i]->plan_iteration();
+      d.u = plans[i].ua;
+      d.u.stats = d_stats.p + i;
+      d.u.work = work.p;
+      d.active = 1;
+      d.list_mode = plans[i].list_mode;
+      d.clear_must = inst[i]->systolic;
     }
+    if (!any) return false;
+    step();
+    for (size_t i = 0; i < k; i++) {
+      if (!running[i]) continue;
+      inst[i]->finish_iteration(plans[i], h_stats[i], 0.0f, float(step_ms.back()));
+      if (inst[i]->stop_after(iters[i], ub[i], stop_threshold)) running[i] = 0;
+      iters[i]++;
+    }
+    return true;
+  }
+
+  // HyperBall::run (hyperball_impl.rs:534-583) for every instance; each one
+  // stops by its own rule, the batch ends when all have stopped.
+  void run(uint64_t upper_bound, double threshold) {
+    begin(upper_bound, threshold);
+    while (advance()) {
+    }
+  }
+
+  // Light/heavy split autotuner for the batch: one threshold for every
+  // instance, chosen by timing the first batched iteration (all instances
+  // dense) under each 64 * 2^k below the longest row, and "no heavy rows".
+  std::vector<std::pair<uint64_t, double>> autotune(uint32_t reps) {
+    if (reps == 0) reps = 1;
+    uint64_t top = 0;
+    for (auto &I : inst)
+      if (!I->heavy_degs.empty()) top = std::max(top, I->heavy_degs.front());
+    std::vector<uint64_t> cand;
+    for (uint64_t t = Instance::kHeavyFloor; t < top; t *= 2) cand.push_back(t);
+    cand.push_back(UINT64_MAX);
+    std::vector<std::pair<uint64_t, double>> table;
+    for (uint64_t t : cand) {
+      for (auto &I : inst) I->set_heavy_degree(t);
+      double best = 1e300;
+      for (uint32_t r = 0; r < reps; r++) {
+        begin(UINT64_MAX, NAN);
+        advance();
+        best = std::min(best, step_ms.back());
+      }
+      table.push_back({t, best});
+    }
+    auto it = std::min_element(table.begin(), table.end(),
+                               [](const auto &a, const auto &b) { return a.second < b.second; });
+    for (auto &I : inst) {
+      I->set_heavy_degree(it->first);
+      I->init();
+    }
+    refresh_heavy();
+    step_ms.clear();
+    return table;
   }
 };
 
@@ -984,6 +1077,22 @@ hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree) {
   });
 }
 
+hb_status hb_autotune(hb_instance *hb, uint32_t reps, uint32_t *thresholds, double *ms,
+                      size_t cap, size_t *len, uint32_t *chosen) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.use_device();
+    const auto table = I.autotune(reps);
+    if (len) *len = table.size();
+    for (size_t k = 0; k < table.size() && k < cap; k++) {
+      if (thresholds)
+        thresholds[k] = table[k].first == UINT64_MAX ? UINT32_MAX : uint32_t(table[k].first);
+      if (ms) ms[k] = table[k].second;
+    }
+    if (chosen) *chosen = I.heavy_degree == UINT64_MAX ? UINT32_MAX : uint32_t(I.heavy_degree);
+  });
+}
+
 // ---- batches ---------------------------------------------------------------
 
 hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transposes,
@@ -1067,6 +1176,23 @@ hb_status hb_batch_run(hb_batch *b, uint64_t upper_bound, double threshold) {
 
 hb_status hb_batch_run_until_done(hb_batch *b) {
   return guard([&] { checked_batch(b)->run(UINT64_MAX, NAN); });
+}
+
+hb_status hb_batch_autotune(hb_batch *b, uint32_t reps, uint32_t *thresholds, double *ms,
+                            size_t cap, size_t *len, uint32_t *chosen) {
+  return guard([&] {
+    Batch &B = *checked_batch(b);
+    HB_HIP(hipSetDevice(B.device));
+    const auto table = B.autotune(reps);
+    if (len) *len = table.size();
+    for (size_t k = 0; k < table.size() && k < cap; k++) {
+      if (thresholds)
+        thresholds[k] = table[k].first == UINT64_MAX ? UINT32_MAX : uint32_t(table[k].first);
+      if (ms) ms[k] = table[k].second;
+    }
+    const uint64_t h = B.inst.front()->heavy_degree;
+    if (chosen) *chosen = h == UINT64_MAX ? UINT32_MAX : uint32_t(h);
+  });
 }
 
 hb_status hb_batch_set_graph(hb_batch *b, int enabled) {

@@ -127,6 +127,8 @@ _SIGNATURES = [
     ("hb_batch_run", _I, [_P, _U64, _D]),
     ("hb_batch_run_until_done", _I, [_P]),
     ("hb_batch_set_graph", _I, [_P, _I]),
+    ("hb_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
+    ("hb_batch_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
     ("hb_batch_step_times", _I, [_P, _P, _SZ, _P]),
 ]
 
@@ -493,6 +495,20 @@ class HyperBall:
     def set_timing(self, on: bool) -> None:
         _check(lib().hb_set_timing(self._h, int(on)))
 
+    def autotune(self, reps: int = 2) -> tuple[int, list[tuple[int, float]]]:
+        """Chooses the light/heavy row split by timing the dense first
+        iteration; returns (chosen threshold, [(threshold, ms), ...]).
+        2**32 - 1 stands for "no heavy rows".  Resets the run state."""
+        cap = 64
+        th = np.zeros(cap, dtype=np.uint32)
+        ms = np.zeros(cap)
+        ln = ctypes.c_size_t()
+        chosen = ctypes.c_uint32()
+        _check(lib().hb_autotune(self._h, reps, _ptr(th), _ptr(ms), cap, ctypes.byref(ln),
+                                 ctypes.byref(chosen)))
+        k = min(ln.value, cap)
+        return chosen.value, [(int(th[i]), float(ms[i])) for i in range(k)]
+
     def set_tuning(self, heavy_degree: int) -> None:
         """Rows longer than heavy_degree go to the workgroup-per-node kernel
         (0 = default, 2**32 - 1 = never)."""
@@ -535,6 +551,19 @@ class HyperBallBatch:
 
     def set_graph(self, on: bool) -> None:
         _check(lib().hb_batch_set_graph(self._h, int(on)))
+
+    def autotune(self, reps: int = 2) -> tuple[int, list[tuple[int, float]]]:
+        """One light/heavy split for all instances, timed on the first
+        batched iteration; returns (chosen, [(threshold, ms), ...])."""
+        cap = 64
+        th = np.zeros(cap, dtype=np.uint32)
+        ms = np.zeros(cap)
+        ln = ctypes.c_size_t()
+        chosen = ctypes.c_uint32()
+        _check(lib().hb_batch_autotune(self._h, reps, _ptr(th), _ptr(ms), cap, ctypes.byref(ln),
+                                       ctypes.byref(chosen)))
+        k = min(ln.value, cap)
+        return chosen.value, [(int(th[i]), float(ms[i])) for i in range(k)]
 
     def step_times(self) -> np.ndarray:
         ln = ctypes.c_size_t()
