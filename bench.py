@@ -87,15 +87,17 @@ def profiled_traffic(args):
 
 def batch_workload(hbm, args):
     """64 independent instances: sizes log-uniform on [10^4, 2*10^6], each
-    graph from the same generator with its own seed, log2m cycling over
-    {6, 8, 10}, 6-bit registers, Jenkins seed per instance."""
+    graph from the same generator with its own seed (1000 + i), log2m drawn
+    uniformly from {6, 8, 10} (seeded), 6-bit registers, Jenkins seed per
+    instance."""
     rng = np.random.default_rng(3)
     ns = np.exp(rng.uniform(np.log(1e4), np.log(2e6), args.batch)).astype(np.int64)
+    log2ms = rng.choice([6, 8, 10], args.batch)
     builders = []
     for i, n in enumerate(ns):
         g = hbm.Graph.web(int(n), 2.1, 3, 10000, 1000, 0.7, 1000 + i)
         gt = g.transpose()
-        builders.append(hbm.HyperBallBuilder.with_hyper_log_log(g, gt, (6, 8, 10)[i % 3])
+        builders.append(hbm.HyperBallBuilder.with_hyper_log_log(g, gt, int(log2ms[i]))
                         .register_bits(args.reg_bits).jenkins(42 + i)
                         .sum_of_distances(True).sum_of_inverse_distances(True))
     return builders
@@ -216,8 +218,8 @@ def rel_err(got, ref):
     err = np.abs(got - ref)
     nz = ref != 0
     err[nz] /= np.abs(ref[nz])
-    return {"max": float(err.max()), "median": float(np.median(err)),
-            "argmax": int(err.argmax())}
+    return {"max": float(err.max()), "p99": float(np.quantile(err, 0.99)),
+            "median": float(np.median(err)), "argmax": int(err.argmax())}
 
 
 def bench_precision(hbm, args):
@@ -261,22 +263,32 @@ def bench_precision(hbm, args):
 def bench_large(hbm, args):
     """One large instance: n = 10^8 nodes, the same generator with out-degree
     Zipf(2.1) on [4, 10000] and a +-10,000 window (about 2e9 arcs), log2m 6,
-    5-bit registers, systolic frontier, f32 Kahan-compensated accumulators."""
+    5-bit registers, systolic frontier, f32 Kahan-compensated accumulators.
+    The transpose is built on the device inside build(); the host counting
+    sort and hb_graph_transpose_device are timed beside it and compared."""
     t = time.perf_counter()
     g = hbm.Graph.web(args.large_nodes, 2.1, 4, 10000, 10000, 0.7, 5)
     print(f"# graph n={g.num_nodes} arcs={g.num_arcs} built in {time.perf_counter() - t:.1f}s",
           file=sys.stderr, flush=True)
     t = time.perf_counter()
     gt = g.transpose()
-    print(f"# transpose built in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    host_t = time.perf_counter() - t
     t = time.perf_counter()
-    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 6).register_bits(5).jenkins(42)
-          .sum_of_distances(True).sum_of_inverse_distances(True)
+    gtd = g.transpose(device=0)
+    dev_t = time.perf_counter() - t
+    same = bool(np.array_equal(gt.offsets, gtd.offsets) and np.array_equal(gt.successors, gtd.successors))
+    print(f"# transpose: host {host_t:.1f}s, device (with both copies) {dev_t:.1f}s, "
+          f"identical {same}", file=sys.stderr, flush=True)
+    del gt, gtd
+    t = time.perf_counter()
+    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, None, 6).transpose_on_device()
+          .register_bits(5).jenkins(42).sum_of_distances(True).sum_of_inverse_distances(True)
           .accumulator_precision(hbm.ACC_F32_KAHAN).build())
     hb.set_timing(True)
-    print(f"# uploaded in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    build_t = time.perf_counter() - t
+    print(f"# instance built (upload + device transpose) in {build_t:.1f}s", file=sys.stderr, flush=True)
     nodes, arcs = g.num_nodes, g.num_arcs
-    del g, gt
+    del g
     for i in range(args.warmup):
         hb.run_until_done()
         print(f"# warmup {i} done", file=sys.stderr, flush=True)
@@ -294,7 +306,9 @@ def bench_large(hbm, args):
     return {"workload": f"web graph n={nodes} arcs={arcs} (+-10000 window, out-degree Zipf(2.1) "
                         "on [4, 10000]), log2m 6, r 5, f32 Kahan accumulators",
             "value": work / elapsed, "unit": "arc-registers/s", "ms_per_step": elapsed * 1e3 / args.steps,
-            "iterations": iters, "union_ms_per_step": union_ms / args.steps}
+            "iterations": iters, "union_ms_per_step": union_ms / args.steps,
+            "setup": {"host_transpose_s": host_t, "device_transpose_with_copies_s": dev_t,
+                      "transpose_identical": same, "build_with_device_transpose_s": build_t}}
 
 
 def main():
@@ -374,6 +388,7 @@ def main():
     work = 0
     bytes_total = 0.0
     union_ms = 0.0
+    dense_bytes, dense_ms = 0.0, 0.0
     launches = 0
     iters = 0
     t0 = time.perf_counter()
@@ -383,8 +398,12 @@ def main():
         iters = len(st)
         for s in st:
             work += s["visited_arcs"] * m
-            bytes_total += algorithmic_bytes(s, g.num_nodes, counter_bytes, 2)
+            b = algorithmic_bytes(s, g.num_nodes, counter_bytes, 2)
+            bytes_total += b
             union_ms += s["union_ms"]
+            if not s["systolic"]:
+                dense_bytes += b
+                dense_ms += s["union_ms"]
             launches += 1
         print(f"# step {i}: {iters} iterations", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
@@ -417,7 +436,9 @@ def main():
                      "kernel": "union step (heavy_kernel + union_kernel, one launch each per "
                                "iteration)", "launches": launches,
                      "avg_launch_ms": union_ms / max(1, launches),
-                     "algorithmic_bytes_per_launch": bytes_total / max(1, launches)},
+                     "algorithmic_bytes_per_launch": bytes_total / max(1, launches),
+                     "dense_iterations_frac": (dense_bytes / (dense_ms * 1e-3) / 1e9 / PEAK_HBM_GBS
+                                               if dense_ms > 0 else None)},
     }
     hb.close()
     del g, gt

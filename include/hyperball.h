@@ -83,6 +83,11 @@ hb_status hb_bvgraph_decoder_offsets(const char *basename, uint64_t *out, uint64
  * `-t` transposed BvGraph the reference loads for systolic iterations
  * (hyperball_impl.rs:204-256). */
 hb_status hb_graph_transpose(const hb_csr *g, hb_graph *out);
+/* The same transpose built on HIP device `device` (in-degree atomics, a scan
+ * and a stable radix sort of (successor, source) pairs) and copied back: the
+ * arrays equal hb_graph_transpose's.  hb_config.device_transpose builds it in
+ * place inside hb_create instead, with no host copy. */
+hb_status hb_graph_transpose_device(const hb_csr *g, int device, hb_graph *out);
 
 /* Synthetic graphs (SplitMix64-seeded xoshiro256**, per-node streams, so the
  * result does not depend on the host thread count).
@@ -126,6 +131,7 @@ typedef struct {
   int sum_of_inverse_distances; /* HyperBallBuilder::sum_of_inverse_distances (:265-268) */
   uint32_t acc_precision; /* HB_ACC_* */
   int device;            /* HIP device ordinal */
+  int device_transpose;  /* with no transpose given: build it on the device (systolic mode) */
 } hb_config;
 
 /* Fills the defaults of HyperBallBuilder::with_hyper_log_log (:91-133):

@@ -234,3 +234,57 @@ def test_autotune_keeps_results(hbm, orc):
     ref.run()
     assert np.array_equal(hb.counters(0), ref.counters(0))
     assert np.array_equal(hb.sum_of_distances(), ref.sum_of_distances())
+
+
+def _transpose_cases(hbm, cnr):
+    rng = np.random.default_rng(11)
+    yield "empty", hbm.Graph.from_arcs(0, [])
+    yield "one node", hbm.Graph.from_arcs(1, [])
+    yield "self loops", hbm.Graph.from_arcs(5, [(0, 0), (1, 1), (1, 0), (4, 2), (2, 4), (3, 3)])
+    yield "no arcs", hbm.Graph.from_arcs(1000, [])
+    yield "star in", hbm.Graph.from_arcs(3000, [(int(v), 0) for v in range(1, 3000)])
+    yield "uniform", hbm.Graph.uniform(50_000, 8, seed=int(rng.integers(1 << 30)))
+    yield "web", hbm.Graph.web(200_000, 2.1, 1, 5000, 500, 0.7, 3)
+    yield "cnr-2000", cnr[0]
+
+
+def test_device_transpose_matches_host(hbm, cnr):
+    """hb_graph_transpose_device (in-degree atomics, scan, stable radix sort)
+    returns exactly the host counting-sort transpose: same offsets, same
+    rows in the same (sorted) order."""
+    for name, g in _transpose_cases(hbm, cnr):
+        th, td = g.transpose(), g.transpose(device=0)
+        assert np.array_equal(th.offsets, td.offsets), name
+        assert np.array_equal(th.successors, td.successors), name
+        # and it is the transpose: (u, v) in G <=> (v, u) in G^T
+        if g.num_arcs:
+            src = np.repeat(np.arange(g.num_nodes, dtype=np.uint64), np.diff(g.offsets.astype(np.int64)))
+            tsrc = np.repeat(np.arange(g.num_nodes, dtype=np.uint64), np.diff(td.offsets.astype(np.int64)))
+            a = np.sort(src * g.num_nodes + g.successors.astype(np.uint64))
+            b = np.sort(td.successors.astype(np.uint64) * g.num_nodes + tsrc)
+            assert np.array_equal(a, b), name
+    # the reference's own transposed file of cnr-2000 (BVGraph cnr-2000-t)
+    td = cnr[0].transpose(device=0)
+    assert np.array_equal(td.offsets, cnr[1].offsets)
+    assert np.array_equal(td.successors, cnr[1].successors)
+
+
+def test_instance_with_device_transpose(hbm):
+    """transpose_on_device(): systolic iterations from a transpose built in
+    place equal those from a host transpose."""
+    g, gt = _web(hbm, 40000, seed=21)
+    runs = []
+    for b in (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8),
+              hbm.HyperBallBuilder.with_hyper_log_log(g, None, 8).transpose_on_device()):
+        hb = b.jenkins(8).sum_of_distances(True).sum_of_inverse_distances(True).build()
+        hb.run_until_done()
+        runs.append((hb.counters(0), hb.sum_of_distances(), hb.harmonic_centralities(),
+                     hb.neighborhood_function(), hb.iteration_stats()))
+    (c1, s1, h1, n1, st1), (c2, s2, h2, n2, st2) = runs
+    assert any(s["systolic"] for s in st2)
+    assert np.array_equal(c1, c2) and np.array_equal(s1, s2) and np.array_equal(h1, h2)
+    assert np.array_equal(n1, n2)
+    for x, y in zip(st1, st2):
+        for k in ("systolic", "local", "pre_local", "checked", "visited_arcs", "transpose_arcs",
+                  "modified"):
+            assert x[k] == y[k], k

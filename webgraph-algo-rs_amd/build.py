@@ -27,6 +27,7 @@ SOURCES = [
     ("hb_graph.cpp", False),
     ("hb_kernels.hip", True),
     ("hb_engine.hip", True),
+    ("hb_transpose.hip", True),
 ]
 
 

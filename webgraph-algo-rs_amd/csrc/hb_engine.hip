@@ -747,7 +747,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     I.seed = cfg->seed;
     I.weight_seed = cfg->weight_seed;
     I.weighted = cfg->weights != nullptr;
-    I.has_t = transpose != nullptr;
+    I.has_t = transpose != nullptr || cfg->device_transpose != 0;
     I.do_sd = cfg->sum_of_distances != 0;
     I.do_hc = cfg->sum_of_inverse_distances != 0;
     I.acc_mode = cfg->acc_precision;
@@ -758,6 +758,10 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     if (transpose) {
       upload(I.toff, transpose->offsets, n + 1, s);
       upload(I.tsucc, transpose->successors, transpose->num_arcs, s);
+    } else if (cfg->device_transpose) {
+      I.toff.alloc(n + 1);
+      I.tsucc.alloc(graph->num_arcs);
+      HB_HIP(launch_transpose(I.off.p, I.succ.p, n, graph->num_arcs, I.toff.p, I.tsucc.p, s));
     }
     std::vector<uint64_t> prefix;
     if (I.weighted) {
@@ -1090,6 +1094,50 @@ hb_status hb_autotune(hb_instance *hb, uint32_t reps, uint32_t *thresholds, doub
       if (ms) ms[k] = table[k].second;
     }
     if (chosen) *chosen = I.heavy_degree == UINT64_MAX ? UINT32_MAX : uint32_t(I.heavy_degree);
+  });
+}
+
+hb_status hb_graph_transpose_device(const hb_csr *g, int device, hb_graph *out) {
+  return guard([&] {
+    if (!out) fail(HB_E_INVALID_ARG, "null output graph");
+    std::memset(out, 0, sizeof(*out));
+    check_csr(g, "graph");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      fail(HB_E_NO_DEVICE, "no HIP device is visible");
+    }
+    if (device < 0 || device >= ndev)
+      fail(HB_E_NO_DEVICE, "device " + std::to_string(device) + " does not exist");
+    HB_HIP(hipSetDevice(device));
+    const uint64_t n = g->num_nodes, a = g->num_arcs;
+    hipStream_t s = nullptr;
+    HB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    DevBuf<uint64_t> off, toff;
+    DevBuf<uint32_t> succ, tsucc;
+    upload(off, g->offsets, n + 1, s);
+    upload(succ, g->successors, a, s);
+    toff.alloc(n + 1);
+    tsucc.alloc(a);
+    HB_HIP(launch_transpose(off.p, succ.p, n, a, toff.p, tsucc.p, s));
+    auto *ho = static_cast<uint64_t *>(std::malloc((n + 1) * sizeof(uint64_t)));
+    auto *hs = static_cast<uint32_t *>(std::malloc(std::max<uint64_t>(1, a) * sizeof(uint32_t)));
+    if (!ho || !hs) {
+      std::free(ho);
+      std::free(hs);
+      fail(HB_E_OOM, "host allocation of the transpose failed");
+    }
+    out->num_nodes = n;
+    out->num_arcs = a;
+    out->offsets = ho;
+    out->successors = hs;
+    HB_HIP(hipMemcpyAsync(ho, toff.p, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    if (a) HB_HIP(hipMemcpyAsync(hs, tsucc.p, a * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HB_HIP(hipStreamSynchronize(s));
   });
 }
 

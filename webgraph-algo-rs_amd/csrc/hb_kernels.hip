@@ -276,6 +276,15 @@ __device__ __forceinline__ uint64_t grab_chunk(uint32_t *work, uint64_t chunks) 
   return UINT64_MAX;
 }
 
+// Orders the LDS traffic of the W lanes of a lane group (W <= 64, one
+// wavefront): writes before the call are visible to reads after it.
+template <int W>
+__device__ __forceinline__ void lane_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Successor ids are read in slices of slice_width<G>() consecutive positions:
 // each lane of a G-lane group reads ids_per_lane<G>() of them, so narrow
 // groups (small counters) still take 16 ids per ballot round.
@@ -331,9 +340,7 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
       cnt += __popcll(mask);
     }
     if (!cnt) continue;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    lane_sync<G>();
     if (!loaded) {
       load_part<D>(prev, own);
 #pragma unroll
@@ -353,8 +360,7 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
 #pragma unroll
         for (int q = 0; q < SPL; q++) slab_max<R>(cur + q * R, y[u] + q * R);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    lane_sync<G>();
   }
 }
 
@@ -421,15 +427,59 @@ __device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int 
   }
 }
 
+// Each wavefront takes chunks of kRoundsPerChunk * (64 / G) positions and
+// gives every lane group one node per round.
+constexpr int kRoundsPerChunk = 8;
+
+template <int G>
+__host__ __device__ constexpr uint64_t chunk_positions() {
+  return uint64_t(kRoundsPerChunk) * (64 / G);
+}
+
+// Per-position node data of a chunk, staged in LDS by stage_chunk.
+struct NodeMeta {
+  uint64_t beg, end;  // CSR row
+  uint32_t v;
+  uint32_t flags;     // kMetaRun | kMetaCheck | kMetaWasMod
+};
+constexpr uint32_t kMetaRun = 1, kMetaCheck = 2, kMetaWasMod = 4;
+
+// Loads the node data of all positions of a chunk in one pass (every lane of
+// the wavefront, independent loads): the order or worklist entry, the
+// frontier bits and the row bounds.  Positions past `count`, nodes neither
+// checked nor modified, and rows left to heavy_kernel get no kMetaRun.
+template <int G>
+__device__ __forceinline__ void stage_chunk(const UnionArgs &a, uint64_t chunk, uint64_t count,
+                                            NodeMeta *meta) {
+  constexpr uint64_t CH = chunk_positions<G>();
+  for (uint32_t p = __lane_id(); p < CH; p += 64) {
+    const uint64_t i = chunk * CH + p;
+    NodeMeta m{0, 0, 0, 0};
+    if (i < count) {
+      const uint32_t v = (a.flags & F_LIST) ? a.list[i] : (a.order ? a.order[i] : uint32_t(i));
+      const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
+      const bool was_mod = test_bit(a.curr_mod, v);
+      m.v = v;
+      if (check || was_mod) {
+        m.beg = a.off[v];
+        m.end = a.off[v + 1];
+        if (m.end - m.beg <= a.heavy_degree)
+          m.flags = kMetaRun | (check ? kMetaCheck : 0u) | (was_mod ? kMetaWasMod : 0u);
+      }
+    }
+    meta[p] = m;
+  }
+  lane_sync<64>();
+}
+
 template <int R, int G, int SPL>
-__device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int gl, Acc &acc_out,
-                                             uint32_t *ids) {
+__device__ __forceinline__ void process_node(const UnionArgs &a, const NodeMeta &m, int gl,
+                                             Acc &acc_out, uint32_t *ids) {
   constexpr int D = R * SPL;
-  const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
-  const bool was_mod = test_bit(a.curr_mod, v);
-  if (!check && !was_mod) return;
-  const uint64_t beg = a.off[v], end = a.off[v + 1];
-  if (end - beg > a.heavy_degree) return;  // handled by heavy_kernel
+  const uint32_t v = m.v;
+  const bool check = m.flags & kMetaCheck;
+  const bool was_mod = m.flags & kMetaWasMod;
+  const uint64_t beg = m.beg, end = m.end;
 
   uint32_t prev[D], cur[D];
   bool loaded = false;
@@ -441,34 +491,25 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, uint32_t v, int
                          check ? end - beg : 0, acc_out);
 }
 
-// Each wavefront takes chunks of kRoundsPerChunk * (64 / G) positions and
-// gives every lane group one node per round.
-constexpr int kRoundsPerChunk = 8;
 
-template <int G>
-__host__ __device__ constexpr uint64_t chunk_positions() {
-  return uint64_t(kRoundsPerChunk) * (64 / G);
-}
-
-// `wave_ids` is the calling wavefront's own LDS region, at least
-// 64 * ids_per_lane<G>() words; its lane groups take consecutive slots.  The
-// region is per wavefront, not per shape: in the batched kernel the
-// wavefronts of one workgroup may run different shapes at the same time.
+// `wave_ids` and `meta` are the calling wavefront's own LDS regions, at least
+// 64 * ids_per_lane<G>() words and chunk_positions<G>() entries; its lane
+// groups take consecutive id slots.  The regions are per wavefront, not per
+// shape: in the batched kernel the wavefronts of one workgroup may run
+// different shapes at the same time.
 template <int R, int G, int SPL>
 __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, uint64_t count,
-                                          uint32_t *wave_ids, Acc &acc) {
+                                          uint32_t *wave_ids, NodeMeta *meta, Acc &acc) {
   constexpr int NPW = 64 / G;
-  constexpr uint64_t CH = chunk_positions<G>();
   const int g = __lane_id() / G, gl = __lane_id() % G;
   uint32_t *ids = wave_ids + g * slice_width<G>();
+  stage_chunk<G>(a, chunk, count, meta);
 #pragma unroll 1
   for (int round = 0; round < kRoundsPerChunk; round++) {
-    const uint64_t i = chunk * CH + uint64_t(round) * NPW + g;
-    if (i < count) {
-      const uint32_t v = (a.flags & F_LIST) ? a.list[i] : (a.order ? a.order[i] : uint32_t(i));
-      process_node<R, G, SPL>(a, v, gl, acc, ids);
-    }
+    const NodeMeta m = meta[round * NPW + g];
+    if (m.flags & kMetaRun) process_node<R, G, SPL>(a, m, gl, acc, ids);
   }
+  lane_sync<64>();  // the next chunk's staging overwrites meta
 }
 
 template <int R, int G, int SPL>
@@ -477,15 +518,18 @@ __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
   const uint64_t chunks = (count + chunk_positions<G>() - 1) / chunk_positions<G>();
   constexpr int kWaveIds = 64 * ids_per_lane<G>();
+  constexpr int kWaveMeta = int(chunk_positions<G>());
   __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
+  __shared__ NodeMeta s_meta[(kBlock / 64) * kWaveMeta];
   uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
+  NodeMeta *meta = s_meta + (threadIdx.x / 64) * kWaveMeta;
   Acc acc;
   for (;;) {
     uint64_t c = 0;
     if (__lane_id() == 0) c = grab_chunk(a.work, chunks);
     c = __shfl(c, 0);
     if (c == UINT64_MAX) break;
-    run_chunk<R, G, SPL>(a, c, count, wave_ids, acc);
+    run_chunk<R, G, SPL>(a, c, count, wave_ids, meta, acc);
   }
   flush_stats(acc, a.stats);
 }
@@ -758,6 +802,13 @@ constexpr int batch_max_d() {
 #undef X
   return d;
 }
+constexpr int batch_wave_meta() {
+  int w = 0;
+#define X(ID, RR, GG, SS) w = cmax(w, int(chunk_positions<GG>()));
+  HB_BATCH_SHAPES(X)
+#undef X
+  return w;
+}
 constexpr int batch_wave_ids() {
   int w = 0;
 #define X(ID, RR, GG, SS) w = cmax(w, 64 * ids_per_lane<GG>());
@@ -862,8 +913,11 @@ __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d,
                                                              const uint64_t *total_chunks,
                                                              uint32_t *work) {
   constexpr int kWaveIds = batch_wave_ids();
+  constexpr int kWaveMeta = batch_wave_meta();
   __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
+  __shared__ NodeMeta s_meta[(kBlock / 64) * kWaveMeta];
   uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
+  NodeMeta *meta = s_meta + (threadIdx.x / 64) * kWaveMeta;
   const uint64_t chunks = *total_chunks;
   Acc acc;
   int current = -1;
@@ -882,7 +936,7 @@ __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d,
     const uint64_t count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
     switch (x.shape) {
 #define X(ID, RR, GG, SS) \
-  case ID: run_chunk<RR, GG, SS>(x.u, c - x.chunk_base, count, wave_ids, acc); break;
+  case ID: run_chunk<RR, GG, SS>(x.u, c - x.chunk_base, count, wave_ids, meta, acc); break;
       HB_BATCH_SHAPES(X)
 #undef X
     }

@@ -147,6 +147,12 @@ hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint
                           uint32_t *list, uint32_t *len, hipStream_t s);
 uint64_t compact_scratch_words(uint64_t n);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s);
+// Transpose of the CSR graph (off[n+1], succ[a]) into caller-allocated
+// toff[n+1], tsucc[a], rows sorted (hb_transpose.hip).  Allocates and frees
+// its own temporaries (about 8 bytes per arc plus the sort's workspace) and
+// returns after the stream has finished them.
+hipError_t launch_transpose(const uint64_t *off, const uint32_t *succ, uint64_t n, uint64_t a,
+                            uint64_t *toff, uint32_t *tsucc, hipStream_t s);
 // Sets bits [0, n) and clears the tail of the last word.
 hipError_t launch_fill_bits(uint32_t *p, uint64_t n, hipStream_t s);
 hipError_t launch_to_reference_layout(const uint32_t *counters, uint64_t node_begin,

@@ -57,7 +57,7 @@ class _Config(ctypes.Structure):
                 ("weights", ctypes.c_void_p), ("weight_seed", ctypes.c_uint64),
                 ("arc_granularity", ctypes.c_uint64), ("sum_of_distances", ctypes.c_int),
                 ("sum_of_inverse_distances", ctypes.c_int), ("acc_precision", ctypes.c_uint32),
-                ("device", ctypes.c_int)]
+                ("device", ctypes.c_int), ("device_transpose", ctypes.c_int)]
 
 
 class _Params(ctypes.Structure):
@@ -89,6 +89,7 @@ _SIGNATURES = [
     ("hb_bvgraph_load_offsets", _I, [ctypes.c_char_p, _U64, _P]),
     ("hb_bvgraph_decoder_offsets", _I, [ctypes.c_char_p, _P, _U64]),
     ("hb_graph_transpose", _I, [_P, _P]),
+    ("hb_graph_transpose_device", _I, [_P, _I, _P]),
     ("hb_graph_generate_uniform", _I, [_U64, _U32, _U64, _P]),
     ("hb_graph_generate_web", _I, [_U64, _D, _U32, _U32, _U64, _D, _U64, _P]),
     ("hb_graph_free", None, [_P]),
@@ -239,10 +240,15 @@ class Graph:
         _check(lib().hb_graph_generate_web(n, alpha, dmin, dmax, window, p_window, seed, ctypes.byref(g)))
         return cls._from_lib(g)
 
-    def transpose(self) -> "Graph":
+    def transpose(self, device: Optional[int] = None) -> "Graph":
+        """The transposed graph, built on the host (counting sort) or, with
+        `device`, on that HIP device (identical arrays)."""
         g = _Graph()
         c = self.csr()
-        _check(lib().hb_graph_transpose(ctypes.byref(c), ctypes.byref(g)))
+        if device is None:
+            _check(lib().hb_graph_transpose(ctypes.byref(c), ctypes.byref(g)))
+        else:
+            _check(lib().hb_graph_transpose_device(ctypes.byref(c), int(device), ctypes.byref(g)))
         return Graph._from_lib(g)
 
 
@@ -276,6 +282,7 @@ class HyperBallBuilder:
         self._discounts: list[Callable[[int], float]] = []
         self._acc = ACC_F64
         self._device = 0
+        self._device_transpose = False
         self._weight_seed = 0
         self._num_elements = 0
         if weights is not None:
@@ -350,6 +357,12 @@ class HyperBallBuilder:
         self._weight_seed = int(s)
         return self
 
+    def transpose_on_device(self, on: bool = True) -> "HyperBallBuilder":
+        """With no transpose given, build it on the device inside build()
+        (enables the systolic mode without a host transpose)."""
+        self._device_transpose = bool(on)
+        return self
+
     def accumulator_precision(self, mode: int) -> "HyperBallBuilder":
         self._acc = int(mode)
         return self
@@ -373,6 +386,7 @@ class HyperBallBuilder:
         cfg.sum_of_inverse_distances = int(self._hc)
         cfg.acc_precision = self._acc
         cfg.device = self._device
+        cfg.device_transpose = int(self._device_transpose)
         return cfg
 
     def build(self) -> "HyperBall":
