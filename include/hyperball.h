@@ -69,6 +69,13 @@ typedef struct {
  * BvGraph::with_basename(..).load() + successor decoding (webgraph 0.2.0;
  * tests/test_hyperball.rs:226-227).  Codes: gamma, delta, unary, zeta_k. */
 hb_status hb_bvgraph_load(const char *basename, hb_graph *out);
+/* The same with an explicit host thread count (0: all cores).  With more than
+ * one thread and a <basename>.offsets file, node ranges are decoded in
+ * parallel from their indexed bit offsets (the random access of
+ * BvGraph::with_basename(..).load() through .offsets / Elias-Fano in webgraph
+ * 0.2.0); references that leave a range are decoded at their own offsets.
+ * One thread: a single sequential pass, no .offsets needed. */
+hb_status hb_bvgraph_load_threads(const char *basename, unsigned threads, hb_graph *out);
 
 /* Decodes <basename>.offsets (gamma-coded bit-offset gaps) into num_nodes+1
  * bit offsets; used to check a decode against the offsets the writer

@@ -101,3 +101,31 @@ def test_web_generator(hbm):
     assert 12 < g.num_arcs / n < 16.5
     g2 = hbm.Graph.web(n, 2.1, 3, 10000, 1000, 0.7, 2)
     assert np.array_equal(g.successors, g2.successors)
+
+
+@pytest.mark.parametrize("name", ["cnr-2000", "cnr-2000-t"])
+def test_bvgraph_parallel_decode_equals_sequential(hbm, name):
+    """Parallel decode through .offsets (ranges balanced by bit length, about
+    8 per thread, so copy references cross range boundaries many times)
+    yields exactly the sequential decode."""
+    base = os.path.join(GOLD, name)
+    seq = hbm.Graph.load_bvgraph(base, threads=1)
+    for t in (2, 3, 8, 64):
+        par = hbm.Graph.load_bvgraph(base, threads=t)
+        assert np.array_equal(seq.offsets, par.offsets), t
+        assert np.array_equal(seq.successors, par.successors), t
+
+
+def test_bvgraph_parallel_decode_rejects_wrong_offsets(hbm, tmp_path):
+    """An .offsets index that does not describe the .graph stream is an error,
+    not a silent misdecode."""
+    import shutil
+    for ext in (".graph", ".properties"):
+        shutil.copy(os.path.join(GOLD, "cnr-2000" + ext), tmp_path / ("g" + ext))
+    shutil.copy(os.path.join(GOLD, "cnr-2000-t.offsets"), tmp_path / "g.offsets")
+    with pytest.raises(hbm.HyperBallError) as e:
+        hbm.Graph.load_bvgraph(str(tmp_path / "g"), threads=4)
+    assert "BVGraph" in e.value.message
+    # one thread ignores the index
+    g = hbm.Graph.load_bvgraph(str(tmp_path / "g"), threads=1)
+    assert g.num_arcs == 3216152

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <exception>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -59,7 +60,8 @@ void parallel_for(uint64_t n, F &&f) {
     return;
   }
   std::vector<std::thread> pool;
-  std::exception_ptr err;
+  std::mutex err_mu;
+  std::exception_ptr err;  // the first failure, rethrown after the join
   uint64_t chunk = (n + t - 1) / t;
   for (unsigned i = 0; i < t; i++) {
     uint64_t b = std::min(n, i * chunk), e = std::min(n, b + chunk);
@@ -67,7 +69,8 @@ void parallel_for(uint64_t n, F &&f) {
       try {
         f(b, e, i);
       } catch (...) {
-        err = std::current_exception();
+        std::lock_guard<std::mutex> l(err_mu);
+        if (!err) err = std::current_exception();
       }
     });
   }

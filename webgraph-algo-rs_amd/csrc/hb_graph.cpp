@@ -14,8 +14,10 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 
 #include "hb_common.hpp"
@@ -30,12 +32,10 @@ namespace {
 // ------------------------------------------------------------------------
 // Bit input, MSB first (dsiutils InputBitStream conventions).
 // ------------------------------------------------------------------------
+// Reads a BitFile's bytes (shared, read-only; one reader per thread).
 class BitReader {
  public:
-  explicit BitReader(std::vector<uint8_t> bytes) : buf_(std::move(bytes)) {
-    nbits_ = uint64_t(buf_.size()) * 8;
-    buf_.resize(buf_.size() + 16, 0);  // padding for 8-byte window loads
-  }
+  BitReader(const uint8_t *data, uint64_t nbits) : buf_(data), nbits_(nbits) {}
   uint64_t position() const { return pos_; }
   void seek(uint64_t bit) { pos_ = bit; }
 
@@ -97,8 +97,19 @@ class BitReader {
   void check(unsigned len) const {
     if (pos_ + len > nbits_) fail(HB_E_INVALID_ARG, "BVGraph: read past the end of the bit stream");
   }
-  std::vector<uint8_t> buf_;
+  const uint8_t *buf_;
   uint64_t nbits_ = 0, pos_ = 0;
+};
+
+// A bit-stream file in memory, padded for the reader's 8-byte windows.
+struct BitFile {
+  std::vector<uint8_t> bytes;
+  uint64_t nbits = 0;
+  explicit BitFile(std::vector<uint8_t> b) : bytes(std::move(b)) {
+    nbits = uint64_t(bytes.size()) * 8;
+    bytes.resize(bytes.size() + 16, 0);
+  }
+  BitReader reader() const { return BitReader(bytes.data(), nbits); }
 };
 
 enum Code { GAMMA, DELTA, UNARY, ZETA };
@@ -195,76 +206,89 @@ uint64_t read_code(BitReader &in, Code c, unsigned zk) {
 // int2nat inverse: 0,1,2,3,4 -> 0,-1,1,-2,2.
 int64_t nat2int(uint64_t x) { return int64_t(x >> 1) ^ -int64_t(x & 1); }
 
+// Decodes the successor list of node x, the reader being positioned at x's
+// code, into `out` (sorted).  ref(r) gives the decoded list of an earlier
+// node r as (pointer, length).  Format: outdegree; if the window is not 0, a
+// reference (x - r); for a reference, a block count and copy/skip block
+// lengths over r's list; then, if min_interval > 0, intervals of consecutive
+// successors; then gap-coded residuals (the first relative to x, signed).
+template <class RefFn>
+void decode_list(BitReader &in, const BvProps &p, uint64_t x, RefFn &&ref,
+                 std::vector<uint32_t> &out, std::vector<uint64_t> &blocks) {
+  const uint64_t n = p.nodes;
+  out.clear();
+  const uint64_t d = read_code(in, p.outdegrees, p.zeta_k);
+  if (d == 0) return;
+  const uint64_t refd = p.window > 0 ? read_code(in, p.references, p.zeta_k) : 0;
+  if (refd > x || refd > p.window) fail(HB_E_INVALID_ARG, "BVGraph: reference out of the window");
+  if (refd > 0) {
+    const auto rl = ref(x - refd);
+    const uint32_t *rs = rl.first;
+    const uint64_t rdeg = rl.second;
+    const uint64_t bc = read_code(in, p.block_count, p.zeta_k);
+    blocks.assign(bc, 0);
+    for (uint64_t i = 0; i < bc; i++)
+      blocks[i] = read_code(in, p.blocks, p.zeta_k) + (i == 0 ? 0 : 1);
+    // Blocks alternate copy / skip, starting with copy; after the last
+    // block the rest of the reference list is copied iff the count is even.
+    uint64_t pos = 0;
+    for (uint64_t i = 0; i < bc; i++) {
+      const uint64_t len = blocks[i];
+      if (pos + len > rdeg) fail(HB_E_INVALID_ARG, "BVGraph: copy block past reference list");
+      if ((i & 1) == 0) out.insert(out.end(), rs + pos, rs + pos + len);
+      pos += len;
+    }
+    if ((bc & 1) == 0) out.insert(out.end(), rs + pos, rs + rdeg);
+  }
+  if (out.size() > d) fail(HB_E_INVALID_ARG, "BVGraph: more copied successors than outdegree");
+  uint64_t extra_count = d - out.size();
+  if (extra_count > 0 && p.min_interval > 0) {
+    const uint64_t ic = in.read_gamma();
+    int64_t prev = 0;
+    for (uint64_t i = 0; i < ic; i++) {
+      int64_t left;
+      if (i == 0) left = nat2int(in.read_gamma()) + int64_t(x);
+      else left = int64_t(in.read_gamma()) + prev + 1;
+      const uint64_t len = in.read_gamma() + p.min_interval;
+      if (left < 0 || uint64_t(left) + len > n || len > extra_count)
+        fail(HB_E_INVALID_ARG, "BVGraph: interval out of range");
+      for (uint64_t k = 0; k < len; k++) out.push_back(uint32_t(left + int64_t(k)));
+      prev = left + int64_t(len);
+      extra_count -= len;
+    }
+  }
+  int64_t prev = 0;
+  for (uint64_t i = 0; i < extra_count; i++) {
+    int64_t s;
+    if (i == 0) s = int64_t(x) + nat2int(read_code(in, p.residuals, p.zeta_k));
+    else s = prev + int64_t(read_code(in, p.residuals, p.zeta_k)) + 1;
+    if (s < 0 || uint64_t(s) >= n) fail(HB_E_INVALID_ARG, "BVGraph: residual out of range");
+    out.push_back(uint32_t(s));
+    prev = s;
+  }
+  std::sort(out.begin(), out.end());
+}
+
 // Sequential decode of every successor list.
 void decode_bvgraph(const std::string &basename, std::vector<uint64_t> &offsets,
                     std::vector<uint32_t> &succ, std::vector<uint64_t> *bit_offsets) {
   BvProps p = read_properties(basename);
-  BitReader in(read_file(basename + ".graph"));
+  const BitFile gf(read_file(basename + ".graph"));
+  BitReader in = gf.reader();
   const uint64_t n = p.nodes;
   offsets.assign(n + 1, 0);
   succ.clear();
   succ.reserve(p.arcs);
   if (bit_offsets) bit_offsets->assign(n + 1, 0);
-  std::vector<int64_t> extra, blocks;
+  std::vector<uint64_t> blocks;
   std::vector<uint32_t> merged;
+  auto ref = [&](uint64_t r) {
+    return std::make_pair(succ.data() + offsets[r], offsets[r + 1] - offsets[r]);
+  };
   for (uint64_t x = 0; x < n; x++) {
     if (bit_offsets) (*bit_offsets)[x] = in.position();
     offsets[x] = succ.size();
-    uint64_t d = read_code(in, p.outdegrees, p.zeta_k);
-    if (d == 0) continue;
-    merged.clear();
-    uint64_t ref = p.window > 0 ? read_code(in, p.references, p.zeta_k) : 0;
-    if (ref > x) fail(HB_E_INVALID_ARG, "BVGraph: reference before node 0");
-    uint64_t copied_total = 0;
-    if (ref > 0) {
-      uint64_t rnode = x - ref;
-      uint64_t rb = offsets[rnode], re = offsets[rnode + 1];
-      uint64_t rdeg = re - rb;
-      uint64_t bc = read_code(in, p.block_count, p.zeta_k);
-      blocks.assign(bc, 0);
-      for (uint64_t i = 0; i < bc; i++)
-        blocks[i] = int64_t(read_code(in, p.blocks, p.zeta_k)) + (i == 0 ? 0 : 1);
-      // Blocks alternate copy / skip, starting with copy; after the last
-      // block the rest of the reference list is copied iff the count is even.
-      uint64_t pos = 0;
-      for (uint64_t i = 0; i < bc; i++) {
-        uint64_t len = uint64_t(blocks[i]);
-        if (pos + len > rdeg) fail(HB_E_INVALID_ARG, "BVGraph: copy block past reference list");
-        if ((i & 1) == 0)
-          for (uint64_t k = 0; k < len; k++) merged.push_back(succ[rb + pos + k]);
-        pos += len;
-      }
-      if ((bc & 1) == 0)
-        for (uint64_t k = pos; k < rdeg; k++) merged.push_back(succ[rb + k]);
-      copied_total = merged.size();
-    }
-    if (copied_total > d) fail(HB_E_INVALID_ARG, "BVGraph: more copied successors than outdegree");
-    uint64_t extra_count = d - copied_total;
-    if (extra_count > 0 && p.min_interval > 0) {
-      uint64_t ic = in.read_gamma();
-      int64_t prev = 0;
-      for (uint64_t i = 0; i < ic; i++) {
-        int64_t left;
-        if (i == 0) left = nat2int(in.read_gamma()) + int64_t(x);
-        else left = int64_t(in.read_gamma()) + prev + 1;
-        uint64_t len = in.read_gamma() + p.min_interval;
-        if (left < 0 || uint64_t(left) + len > n || len > extra_count)
-          fail(HB_E_INVALID_ARG, "BVGraph: interval out of range");
-        for (uint64_t k = 0; k < len; k++) merged.push_back(uint32_t(left + int64_t(k)));
-        prev = left + int64_t(len);
-        extra_count -= len;
-      }
-    }
-    int64_t prev = 0;
-    for (uint64_t i = 0; i < extra_count; i++) {
-      int64_t s;
-      if (i == 0) s = int64_t(x) + nat2int(read_code(in, p.residuals, p.zeta_k));
-      else s = prev + int64_t(read_code(in, p.residuals, p.zeta_k)) + 1;
-      if (s < 0 || uint64_t(s) >= n) fail(HB_E_INVALID_ARG, "BVGraph: residual out of range");
-      merged.push_back(uint32_t(s));
-      prev = s;
-    }
-    std::sort(merged.begin(), merged.end());
+    decode_list(in, p, x, ref, merged, blocks);
     succ.insert(succ.end(), merged.begin(), merged.end());
   }
   offsets[n] = succ.size();
@@ -272,6 +296,114 @@ void decode_bvgraph(const std::string &basename, std::vector<uint64_t> &offsets,
   if (succ.size() != p.arcs)
     fail(HB_E_INVALID_ARG, "BVGraph: decoded " + std::to_string(succ.size()) +
                                " arcs, properties say " + std::to_string(p.arcs));
+}
+
+// Bit offset of every node's code in <basename>.graph, and the stream's end,
+// from <basename>.offsets (gap-coded with the OFFSETS code).
+std::vector<uint64_t> read_bit_offsets(const std::string &basename, const BvProps &p) {
+  const BitFile of(read_file(basename + ".offsets"));
+  BitReader in = of.reader();
+  std::vector<uint64_t> out(p.nodes + 1);
+  uint64_t acc = 0;
+  for (uint64_t i = 0; i <= p.nodes; i++) {
+    acc += read_code(in, p.offsets, p.zeta_k);
+    out[i] = acc;
+  }
+  return out;
+}
+
+// Parallel decode through the .offsets index: node ranges balanced by bit
+// length are decoded by separate threads.  A reference to a node before a
+// thread's range is resolved by decoding that node at its indexed bit offset
+// (recursively through its own references), with a per-thread cache.  Every
+// range must end exactly at the next range's indexed offset.
+void decode_bvgraph_parallel(const std::string &basename, unsigned threads,
+                             std::vector<uint64_t> &offsets, std::vector<uint32_t> &succ) {
+  const BvProps p = read_properties(basename);
+  const uint64_t n = p.nodes;
+  const std::vector<uint64_t> bits = read_bit_offsets(basename, p);
+  const BitFile gf(read_file(basename + ".graph"));
+  if (bits[n] > gf.nbits) fail(HB_E_INVALID_ARG, "BVGraph: .offsets points past the .graph stream");
+  for (uint64_t x = 0; x < n; x++)
+    if (bits[x + 1] < bits[x]) fail(HB_E_INVALID_ARG, "BVGraph: .offsets are not monotone");
+  const uint64_t want = std::max<uint64_t>(1, std::min<uint64_t>(n, uint64_t(threads) * 8));
+  std::vector<uint64_t> bounds{0};
+  for (uint64_t c = 1; c < want; c++) {
+    const uint64_t target = bits[n] / want * c;
+    const uint64_t x = uint64_t(std::lower_bound(bits.begin(), bits.begin() + n, target) - bits.begin());
+    if (x > bounds.back() && x < n) bounds.push_back(x);
+  }
+  bounds.push_back(n);
+  const size_t chunks = bounds.size() - 1;
+  std::vector<std::vector<uint64_t>> loff(chunks);
+  std::vector<std::vector<uint32_t>> lsucc(chunks);
+  std::atomic<size_t> next{0};
+  std::mutex err_mu;
+  std::exception_ptr err;
+  auto worker = [&] {
+    try {
+      for (size_t c; (c = next.fetch_add(1)) < chunks;) {
+        const uint64_t b = bounds[c], e = bounds[c + 1];
+        std::vector<uint64_t> &lo = loff[c];
+        std::vector<uint32_t> &ls = lsucc[c];
+        lo.reserve(e - b + 1);
+        std::map<uint64_t, std::vector<uint32_t>> outside;
+        std::vector<uint64_t> blocks;
+        std::function<std::pair<const uint32_t *, uint64_t>(uint64_t)> ref =
+            [&](uint64_t r) -> std::pair<const uint32_t *, uint64_t> {
+          if (r >= b) return {ls.data() + lo[r - b], lo[r - b + 1] - lo[r - b]};
+          auto it = outside.find(r);
+          if (it == outside.end()) {
+            BitReader rin = gf.reader();
+            rin.seek(bits[r]);
+            std::vector<uint32_t> lst;
+            std::vector<uint64_t> blk;
+            decode_list(rin, p, r, ref, lst, blk);
+            it = outside.emplace(r, std::move(lst)).first;
+          }
+          return {it->second.data(), it->second.size()};
+        };
+        BitReader in = gf.reader();
+        in.seek(bits[b]);
+        std::vector<uint32_t> merged;
+        for (uint64_t x = b; x < e; x++) {
+          lo.push_back(ls.size());  // also the end of x - 1's list, for ref()
+          if (in.position() != bits[x])
+            fail(HB_E_INVALID_ARG, "BVGraph: .offsets disagree with the .graph stream at node " +
+                                       std::to_string(x));
+          decode_list(in, p, x, ref, merged, blocks);
+          ls.insert(ls.end(), merged.begin(), merged.end());
+        }
+        lo.push_back(ls.size());
+        if (in.position() != bits[e])
+          fail(HB_E_INVALID_ARG, "BVGraph: .offsets disagree with the .graph stream at node " +
+                                     std::to_string(e));
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> l(err_mu);
+      if (!err) err = std::current_exception();
+    }
+  };
+  std::vector<std::thread> pool;
+  const unsigned t = std::max(1u, std::min<unsigned>(threads, unsigned(chunks)));
+  for (unsigned i = 0; i < t; i++) pool.emplace_back(worker);
+  for (auto &th : pool) th.join();
+  if (err) std::rethrow_exception(err);
+  offsets.assign(n + 1, 0);
+  uint64_t total = 0;
+  for (size_t c = 0; c < chunks; c++) total += lsucc[c].size();
+  if (total != p.arcs)
+    fail(HB_E_INVALID_ARG, "BVGraph: decoded " + std::to_string(total) + " arcs, properties say " +
+                               std::to_string(p.arcs));
+  succ.assign(total, 0);
+  uint64_t base = 0;
+  for (size_t c = 0; c < chunks; c++) {
+    const uint64_t b = bounds[c], e = bounds[c + 1];
+    for (uint64_t x = b; x < e; x++) offsets[x] = base + loff[c][x - b];
+    std::copy(lsucc[c].begin(), lsucc[c].end(), succ.begin() + base);
+    base += lsucc[c].size();
+  }
+  offsets[n] = base;
 }
 
 template <class T>
@@ -325,14 +457,23 @@ extern "C" {
 
 const char *hb_last_error(void) { return hb::g_last_error.c_str(); }
 
-hb_status hb_bvgraph_load(const char *basename, hb_graph *out) {
+hb_status hb_bvgraph_load_threads(const char *basename, unsigned threads, hb_graph *out) {
   return guard([&] {
     if (!basename || !out) fail(HB_E_INVALID_ARG, "null argument");
     std::vector<uint64_t> off;
     std::vector<uint32_t> succ;
-    decode_bvgraph(basename, off, succ, nullptr);
+    if (threads == 0) threads = host_threads();
+    const std::string base(basename);
+    if (threads > 1 && std::ifstream(base + ".offsets").good())
+      decode_bvgraph_parallel(base, threads, off, succ);
+    else
+      decode_bvgraph(base, off, succ, nullptr);
     fill_graph(out, off.size() - 1, std::move(off), std::move(succ));
   });
+}
+
+hb_status hb_bvgraph_load(const char *basename, hb_graph *out) {
+  return hb_bvgraph_load_threads(basename, 0, out);
 }
 
 hb_status hb_bvgraph_decoder_offsets(const char *basename, uint64_t *out, uint64_t cap) {
@@ -350,12 +491,10 @@ hb_status hb_bvgraph_load_offsets(const char *basename, uint64_t num_nodes, uint
   return guard([&] {
     if (!basename || !out) fail(HB_E_INVALID_ARG, "null argument");
     BvProps p = read_properties(basename);
-    BitReader in(read_file(std::string(basename) + ".offsets"));
-    uint64_t acc = 0;
-    for (uint64_t i = 0; i <= num_nodes; i++) {
-      acc += read_code(in, p.offsets, p.zeta_k);
-      out[i] = acc;
-    }
+    if (num_nodes != p.nodes)
+      fail(HB_E_INVALID_ARG, "BVGraph: the graph has " + std::to_string(p.nodes) + " nodes");
+    const std::vector<uint64_t> bits = read_bit_offsets(basename, p);
+    std::memcpy(out, bits.data(), bits.size() * sizeof(uint64_t));
   });
 }
 

@@ -89,6 +89,7 @@ _SIGNATURES = [
     ("hb_bvgraph_load_offsets", _I, [ctypes.c_char_p, _U64, _P]),
     ("hb_bvgraph_decoder_offsets", _I, [ctypes.c_char_p, _P, _U64]),
     ("hb_graph_transpose", _I, [_P, _P]),
+    ("hb_bvgraph_load_threads", _I, [_P, _U32, _P]),
     ("hb_graph_transpose_device", _I, [_P, _I, _P]),
     ("hb_graph_generate_uniform", _I, [_U64, _U32, _U64, _P]),
     ("hb_graph_generate_web", _I, [_U64, _D, _U32, _U32, _U64, _D, _U64, _P]),
@@ -222,9 +223,14 @@ class Graph:
         return cls(off, arcs[:, 1].astype(np.uint32) if len(arcs) else np.zeros(0, np.uint32))
 
     @classmethod
-    def load_bvgraph(cls, basename: str) -> "Graph":
+    def load_bvgraph(cls, basename: str, threads: Optional[int] = None) -> "Graph":
+        """Decodes <basename>.graph (BVGraph v0); threads None = all cores,
+        in parallel through <basename>.offsets when it exists."""
         g = _Graph()
-        _check(lib().hb_bvgraph_load(basename.encode(), ctypes.byref(g)))
+        if threads is None:
+            _check(lib().hb_bvgraph_load(basename.encode(), ctypes.byref(g)))
+        else:
+            _check(lib().hb_bvgraph_load_threads(basename.encode(), int(threads), ctypes.byref(g)))
         return cls._from_lib(g)
 
     @classmethod
