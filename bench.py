@@ -33,12 +33,22 @@ def _load(name, path):
     return mod
 
 
-def algorithmic_bytes(s, n, counter_bytes, outputs):
+def algorithmic_bytes(s, n, counter_bytes, slab_bytes, outputs, delta_aware=True):
     """Per-iteration bytes of the no-reuse model: successor counters read,
     own counter read and written, stale copies, CSR and transpose ids,
-    offsets, accumulators read-modify-write and the four frontier bitmaps."""
+    offsets, accumulators read-modify-write and the four frontier bitmaps.
+    delta_aware: successor reads are the slabs actually gathered (delta
+    gathering reads only the slabs that changed), plus the 4-byte change
+    mask read per visited arc in such iterations and the 4-byte mask written
+    per modified node; otherwise C per visited arc."""
     C = counter_bytes
-    return (C * (s["visited_arcs"] + 2 * s["checked"] + 2 * s["copied"])
+    if delta_aware:
+        succ = s["gathered_slabs"] * slab_bytes + 4 * s["modified"]
+        if s["delta_gathering"]:
+            succ += 4 * s["visited_arcs"]
+    else:
+        succ = C * s["visited_arcs"]
+    return (succ + C * (2 * s["checked"] + 2 * s["copied"])
             + 4 * (s["scanned_arcs"] + s["transpose_arcs"]) + 8 * s["checked"]
             + 16 * s["modified"] * outputs + 4 * ((n + 7) // 8))
 
@@ -380,6 +390,7 @@ def main():
     hb.set_timing(True)
     p = hb.params()
     m, counter_bytes = p["num_registers"], p["words_per_counter"] * 8
+    slab_bytes = 4 * p["register_bits"]  # 32 registers
 
     for i in range(args.warmup):
         hb.run_until_done()
@@ -387,6 +398,7 @@ def main():
 
     work = 0
     bytes_total = 0.0
+    bytes_no_delta = 0.0
     union_ms = 0.0
     dense_bytes, dense_ms = 0.0, 0.0
     launches = 0
@@ -398,8 +410,10 @@ def main():
         iters = len(st)
         for s in st:
             work += s["visited_arcs"] * m
-            b = algorithmic_bytes(s, g.num_nodes, counter_bytes, 2)
+            b = algorithmic_bytes(s, g.num_nodes, counter_bytes, slab_bytes, 2)
             bytes_total += b
+            bytes_no_delta += algorithmic_bytes(s, g.num_nodes, counter_bytes, slab_bytes, 2,
+                                                delta_aware=False)
             union_ms += s["union_ms"]
             if not s["systolic"]:
                 dense_bytes += b
@@ -438,7 +452,11 @@ def main():
                      "avg_launch_ms": union_ms / max(1, launches),
                      "algorithmic_bytes_per_launch": bytes_total / max(1, launches),
                      "dense_iterations_frac": (dense_bytes / (dense_ms * 1e-3) / 1e9 / PEAK_HBM_GBS
-                                               if dense_ms > 0 else None)},
+                                               if dense_ms > 0 else None),
+                     "bytes_model": "slabs actually gathered (delta gathering); the model with C "
+                                    "bytes per visited arc gives no_delta_model_frac",
+                     "no_delta_model_frac": (bytes_no_delta / (union_ms * 1e-3) / 1e9 / PEAK_HBM_GBS
+                                             if union_ms > 0 else None)},
     }
     hb.close()
     del g, gt

@@ -217,6 +217,9 @@ typedef struct {
   double nf;              /* neighbourhood-function value pushed */
   double union_ms;        /* device time of the union kernel(s), when timing is on */
   double iteration_ms;    /* device time of the whole iteration, when timing is on */
+  uint64_t gathered_slabs;/* successor slabs (32 registers each) actually read */
+  uint32_t delta_gathering; /* 1 when only the slabs that changed in the previous iteration
+                               were gathered (chosen per iteration) */
 } hb_iteration_stats;
 hb_status hb_num_iterations(hb_instance *hb, uint64_t *out);
 hb_status hb_get_iteration_stats(hb_instance *hb, uint64_t iteration, hb_iteration_stats *out);
@@ -226,6 +229,12 @@ hb_status hb_set_timing(hb_instance *hb, int enabled);
  * default); for the autotuner.  heavy_degree: rows with more successors than
  * this go to the workgroup-per-node kernel. */
 hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree);
+/* Delta gathering: read only the slabs (32 registers) of a modified successor
+ * that changed in the previous iteration; exact, since every counter already
+ * dominates its successors' previous counters.  mode 0 (default): per
+ * iteration, when counters have at least 1024 registers and the expected
+ * share of changed slabs is below 0.7; 1: always; 2: never. */
+hb_status hb_set_delta_gathering(hb_instance *hb, int mode);
 /* Autotuner of that split (in the role of the reference's arc_granularity
  * tuning knob, hyperball_impl.rs:271-274, which sizes CPU work units): times
  * the first, dense iteration under each threshold 64 * 2^k that changes the

@@ -83,10 +83,19 @@ def build(drivers=None, verbose=False):
         objs.append(obj)
     if drivers is None:
         drivers = sorted(f for f in os.listdir(HERE) if f.endswith("_check.cpp"))
+    # the oracle's C restatement, for the drivers that check against it
+    oracle_src = os.path.join(REPO, "oracle", "hb_oracle.c")
+    oracle_obj = os.path.join(OUT, "hb_oracle.o")
+    subprocess.run(["gcc", "-std=c11", "-g", "-O1", "-fno-omit-frame-pointer",
+                    "-fsanitize=address,undefined", "-ffp-contract=off", "-w", "-fopenmp",
+                    "-c", oracle_src, "-o", oracle_obj], check=True)
     exes = []
     for d in drivers:
         exe = os.path.join(OUT, d[:-4])
-        subprocess.run(["g++", *FLAGS, *inc, os.path.join(HERE, d), *objs, "-o", exe], check=True)
+        extra = (["-I", os.path.join(REPO, "oracle"), oracle_obj, "-fopenmp"]
+                 if d.startswith("oracle") else [])
+        subprocess.run(["g++", *FLAGS, *inc, os.path.join(HERE, d), *objs, *extra, "-lm", "-o", exe],
+                       check=True)
         exes.append(exe)
     return exes, launches
 

@@ -7,7 +7,7 @@ single-instance result fails here, without a GPU.
 
 The batch mixes all six batched shapes (log2m 6, 8, 10 with 5- and 6-bit
 registers), so wavefronts of one workgroup run different shapes at once;
-rows up to 900 successors exercise the workgroup-per-node kernel."""
+rows up to 600 successors exercise the workgroup-per-node kernel."""
 import os
 import shutil
 import subprocess
@@ -32,8 +32,28 @@ def emu_exes():
 @pytest.mark.parametrize("graph,heavy,tune", [(1, 0, 1), (0, 64, 0)])
 def test_emulated_batch_matches_single_instances(emu_exes, graph, heavy, tune):
     """(graph replay, batch autotuner) and (direct launches, heavy rows above 64)."""
-    args = [emu_exes["batch_check"], "6", str(graph), "7", "200", "600", "900", str(heavy),
+    args = [emu_exes["batch_check"], "6", str(graph), "7", "150", "400", "600", str(heavy),
             str(tune)]
     r = subprocess.run(args, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:] + r.stderr[-6000:]
     assert r.stdout.count("identical") == 6
+
+
+@pytest.mark.parametrize("log2m,r,n,dmax,seed,heavy,delta", [
+    (6, 5, 500, 400, 4, 64, 1),     # 2 slabs, G = 2, heavy rows, delta gathering forced
+    (8, 6, 400, 300, 5, 64, 1),     # 8 slabs, G = 8
+    (10, 6, 300, 250, 3, 0, 0),     # 32 slabs: one change-mask bit each; the default rule
+    (10, 5, 300, 250, 8, 64, 2),    # no delta gathering
+    (12, 5, 200, 150, 6, 0, 1),     # 128 slabs: four per change-mask bit
+    (5, 6, 300, 250, 7, 64, 1),     # one slab, G = 1
+])
+def test_emulated_engine_matches_oracle(emu_exes, log2m, r, n, dmax, seed, heavy, delta):
+    """Every iteration (standard, systolic, local, pre-local) of the emulated
+    kernels, with and without delta gathering, equals the oracle's
+    restatement of the reference engine: registers, modified bits, mode; then
+    sums of distances."""
+    args = [emu_exes["oracle_check"], str(log2m), str(r), str(n), str(dmax), str(seed), str(heavy),
+            str(delta)]
+    res = subprocess.run(args, capture_output=True, text=True, timeout=900)
+    assert res.returncode == 0 and res.stdout.strip().endswith("OK"), res.stdout[-3000:] + res.stderr[-6000:]
+    assert "DIFFERENT" not in res.stdout

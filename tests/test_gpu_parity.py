@@ -27,13 +27,16 @@ def _tiny_graphs(hbm):
             "self_loops": loops}
 
 
-def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_000, heavy=None):
+def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_000, heavy=None,
+                  delta=None):
     b = hbm.HyperBallBuilder.with_hyper_log_log(g, gt, log2m).register_bits(reg_bits)
     if jenkins is not None:
         b = b.jenkins(jenkins)
     hb = b.build()
     if heavy is not None:
         hb.set_tuning(heavy)
+    if delta is not None:
+        hb.set_delta_gathering(delta)
     hll = orc.Hll(log2m, reg_bits, g.num_nodes, orc.HASH_JENKINS if jenkins is not None else orc.HASH_SIPHASH13,
                   jenkins or 0)
     ref = orc.Jacobi(hll, g.offsets, g.successors)
@@ -142,3 +145,23 @@ def test_heavy_rows_bit_exact(hbm, orc, heavy, log2m, reg_bits):
     arcs += [(int(u), int((u + 1) % n)) for u in range(0, n, 3)]
     g = hbm.Graph.from_arcs(n, arcs)
     _run_lockstep(hbm, orc, g, g.transpose(), log2m, reg_bits, jenkins=5, heavy=heavy)
+
+
+@pytest.mark.parametrize("delta", [1, 2])
+@pytest.mark.parametrize("log2m,reg_bits", [(5, 6), (6, 5), (8, 6), (10, 6), (12, 5)])
+def test_delta_gathering_bit_exact(hbm, orc, delta, log2m, reg_bits):
+    """Delta gathering forced on (1) and off (2): every iteration bit-exact
+    against the Jacobi oracle, heavy rows included; with it on, only a part
+    of the successor slabs is read."""
+    g = hbm.Graph.web(20000, 2.1, 1, 2000, 300, 0.7, 13)
+    it, stats = _run_lockstep(hbm, orc, g, g.transpose(), log2m, reg_bits, jenkins=3, heavy=64,
+                              delta=delta)
+    assert it > 5
+    assert all(s["delta_gathering"] == (delta == 1) for s in stats)
+    slabs = max(1, (1 << log2m) // 32)
+    gathered = sum(s["gathered_slabs"] for s in stats)
+    total = sum(s["visited_arcs"] for s in stats) * slabs
+    if delta == 2 or slabs == 1:
+        assert gathered == total
+    else:
+        assert gathered < total

@@ -110,6 +110,19 @@ struct Instance {
   DevBuf<uint64_t> off, toff, d_weights, d_wprefix;
   DevBuf<uint32_t> succ, tsucc;
   DevBuf<uint32_t> cnt[2], mod[2], must[2];
+  DevBuf<uint32_t> chg[2];  // change masks, swapped with the modified bitmaps
+  uint32_t chg_shift = 0;   // log2 slabs per change-mask bit
+  // Delta gathering (read only the changed slabs of successors) pays when
+  // counters are large and few of their slabs changed: it costs a 4-byte mask
+  // read per modified successor and splits each lane's load by slab.
+  // Measured on the web graph at log2m 10, r 6: a dense iteration takes about
+  // 14.2 + 6.6 f ms with it against 19.1 ms without, f being the fraction of
+  // slabs gathered, and nothing is gained below log2m 10.
+  static constexpr uint32_t kDeltaMinLog2m = 10;
+  static constexpr double kDeltaMaxFraction = 0.7;
+  uint64_t last_changed = 0;  // change-mask bits set by the last iteration
+  bool delta = false;         // the last planned iteration gathers deltas
+  int delta_mode = 0;         // 0: by the rule above, 1: always, 2: never
   DevBuf<double> cache, lc;
   DevBuf<uint8_t> sd, hc;
   std::vector<std::unique_ptr<DevBuf<uint8_t>>> dc;
@@ -214,6 +227,8 @@ struct Instance {
     s.weights = weighted ? d_weights.p : nullptr;
     s.weight_prefix = weighted ? d_wprefix.p : nullptr;
     s.weight_seed = weight_seed;
+    s.chg = chg[0].p;
+    s.chg_shift = chg_shift;
     HB_HIP(launch_seed(s, stream));
     HB_HIP(launch_count_all(int(r), int(log2m), cnt[0].p, n, amm, m, lc.p, cache.p, stream));
     if (do_sd) fill_zero(sd.p, n * acc_bytes());
@@ -225,6 +240,7 @@ struct Instance {
     fill_zero(must[1].p, bitmap_words * 4);
     HB_HIP(hipStreamSynchronize(stream));
     cur = mcur = mustcur = 0;
+    last_changed = 0;
     iteration = 0;
     systolic = local = pre_local = false;
     modified = 0;
@@ -267,6 +283,18 @@ struct Instance {
       flags |= F_CHECK_ALL;  // standard, or the first systolic iteration (all nodes checked)
     }
     if (p.list_mode) flags |= F_LIST;
+    {
+      // expected fraction of successor slabs to gather: one non-zero slab per
+      // counter after unweighted seeding; then the mean share of changed
+      // slabs over the nodes the last iteration modified
+      const double bits = double(std::min<uint32_t>(32, m / 32));
+      const double est = iteration == 0 ? (weighted ? 1.0 : 1.0 / bits)
+                         : modified_prev ? double(last_changed) / (double(modified_prev) * bits)
+                                         : 1.0;
+      delta = delta_mode == 1 ||
+              (delta_mode == 0 && log2m >= kDeltaMinLog2m && est < kDeltaMaxFraction);
+      if (delta) flags |= F_DELTA;
+    }
 
     UnionArgs &ua = p.ua;
     ua.off = off.p;
@@ -302,6 +330,9 @@ struct Instance {
     ua.heavy_degree = heavy_count ? heavy_degree : UINT64_MAX;
     ua.heavy_list = d_heavy.p;
     ua.heavy_count = heavy_count;
+    ua.chg_cur = chg[mcur].p;
+    ua.chg_next = chg[1 - mcur].p;
+    ua.chg_shift = chg_shift;
 
     int lanes = 1, spl = 1;
     union_shape(int(r), int(log2m), &lanes, &spl);
@@ -328,6 +359,9 @@ struct Instance {
     hs.transpose_arcs = st.tscanned;
     hs.copied = st.copied;
     hs.modified = st.modified;
+    hs.gathered_slabs = st.gathered;
+    hs.delta_gathering = delta;
+    last_changed = st.changed;
     hs.union_ms = union_ms;
     hs.iteration_ms = iteration_ms;
 
@@ -488,8 +522,8 @@ struct Batch {
   Stats *h_stats = nullptr;     // pinned
   DevBuf<uint32_t> work;
   DevBuf<uint64_t> total_chunks;
-  uint32_t prep_blocks = 1, compact_blocks = 1, union_grid = 64, heavy_grid = 0;
-  uint64_t total_heavy = 0;
+  DevBuf<uint32_t> shape_inst, shape_off;  // instances grouped by shape
+  BatchLaunch L{};
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -517,24 +551,30 @@ struct Batch {
     graph = nullptr;
   }
 
+  std::vector<uint32_t> h_shape_inst;
+
+  // Heavy-row ranges, shape by shape (the heavy kernels' launch sizes are
+  // captured in the graph: a change drops it).
   void refresh_heavy() {
-    uint64_t acc = 0;
-    for (size_t i = 0; i < inst.size(); i++) {
-      h_desc[i].heavy_base = acc;
-      acc += inst[i]->heavy_count;
-      h_desc[i].heavy_end = acc;
+    for (int sh = 0; sh < kBatchShapes; sh++) {
+      uint64_t acc = 0;
+      for (uint32_t j = L.first[sh]; j < L.first[sh] + L.count[sh]; j++) {
+        const uint32_t i = h_shape_inst[j];
+        h_desc[i].heavy_base = acc;
+        acc += inst[i]->heavy_count;
+        h_desc[i].heavy_end = acc;
+      }
+      const uint32_t hg = uint32_t(std::min<uint64_t>(acc, Instance::kHeavyGridCap));
+      if (hg != L.heavy_grid[sh] || acc != L.total_heavy[sh]) drop_graph();
+      L.total_heavy[sh] = acc;
+      L.heavy_grid[sh] = hg;
     }
-    const uint32_t hg = uint32_t(std::min<uint64_t>(acc, Instance::kHeavyGridCap));
-    if (hg != heavy_grid || acc != total_heavy) drop_graph();
-    total_heavy = acc;
-    heavy_grid = hg;
   }
 
   void enqueue_step() {
     const int k = int(inst.size());
     HB_HIP(hipMemcpyAsync(d_desc.p, h_desc, k * sizeof(BatchDesc), hipMemcpyHostToDevice, stream));
-    HB_HIP(launch_batch_iteration(d_desc.p, k, d_stats.p, work.p, total_chunks.p, prep_blocks,
-                                  compact_blocks, union_grid, heavy_grid, total_heavy, stream));
+    HB_HIP(launch_batch_iteration(L, stream));
     HB_HIP(hipMemcpyAsync(h_stats, d_stats.p, k * sizeof(Stats), hipMemcpyDeviceToHost, stream));
   }
 
@@ -741,6 +781,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     I.r = r;
     I.W = uint32_t(m * r / 64);
     I.cdw = uint32_t(m * r / 32);
+    while ((m / 32) >> I.chg_shift > 32) I.chg_shift++;
     I.sentinel = uint64_t(1) << ((1u << r) - 2);
     I.amm = alpha_m_m(cfg->log2m);
     I.hash = cfg->hash;
@@ -797,6 +838,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
       I.cnt[k].alloc(size_t(n) * I.cdw);
       I.mod[k].alloc(I.bitmap_words);
       I.must[k].alloc(I.bitmap_words);
+      I.chg[k].alloc(n);
     }
     I.cache.alloc(n);
     if (I.do_sd) I.sd.alloc(n * I.acc_bytes());
@@ -1081,6 +1123,13 @@ hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree) {
   });
 }
 
+hb_status hb_set_delta_gathering(hb_instance *hb, int mode) {
+  return guard([&] {
+    if (mode < 0 || mode > 2) fail(HB_E_INVALID_ARG, "delta gathering mode must be 0, 1 or 2");
+    checked(hb)->delta_mode = mode;
+  });
+}
+
 hb_status hb_autotune(hb_instance *hb, uint32_t reps, uint32_t *thresholds, double *ms,
                       size_t cap, size_t *len, uint32_t *chosen) {
   return guard([&] {
@@ -1183,8 +1232,8 @@ hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transpos
     b->h_stats = static_cast<Stats *>(p);
     b->d_desc.alloc(k);
     b->d_stats.alloc(k);
-    b->work.alloc(kWorkCounters * kWorkStride);
-    b->total_chunks.alloc(1);
+    b->work.alloc(kBatchWorkWords);
+    b->total_chunks.alloc(kBatchShapes);
     for (size_t i = 0; i < k; i++) {
       Instance &I = *b->inst[i];
       BatchDesc &d = b->h_desc[i];
@@ -1193,9 +1242,33 @@ hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transpos
       d.n = I.n;
       d.bsum = I.bsum.p;
     }
-    b->prep_blocks = uint32_t(std::min<uint64_t>(64, (max_words + kBlock - 1) / kBlock));
-    b->compact_blocks = uint32_t(std::max<uint64_t>(1, ((max_nodes + 31) / 32 + kBlock - 1) / kBlock));
-    b->union_grid = b->inst[0]->max_grid;
+    // group the instances by shape, in instance order within a shape
+    std::vector<uint32_t> off(kBatchShapes + 1, 0);
+    for (size_t i = 0; i < k; i++) off[b->h_desc[i].shape + 1]++;
+    for (int sh = 0; sh < kBatchShapes; sh++) off[sh + 1] += off[sh];
+    b->h_shape_inst.assign(k, 0);
+    {
+      std::vector<uint32_t> pos(off.begin(), off.end() - 1);
+      for (size_t i = 0; i < k; i++) b->h_shape_inst[pos[b->h_desc[i].shape]++] = uint32_t(i);
+    }
+    upload(b->shape_inst, b->h_shape_inst.data(), k, b->stream);
+    upload(b->shape_off, off.data(), off.size(), b->stream);
+    HB_HIP(hipStreamSynchronize(b->stream));
+    BatchLaunch &L = b->L;
+    L.d = b->d_desc.p;
+    L.k = int(k);
+    L.stats = b->d_stats.p;
+    L.work = b->work.p;
+    L.total_chunks = b->total_chunks.p;
+    L.shape_inst = b->shape_inst.p;
+    L.shape_off = b->shape_off.p;
+    for (int sh = 0; sh < kBatchShapes; sh++) {
+      L.first[sh] = off[sh];
+      L.count[sh] = off[sh + 1] - off[sh];
+    }
+    L.prep_blocks = uint32_t(std::min<uint64_t>(64, (max_words + kBlock - 1) / kBlock));
+    L.compact_blocks = uint32_t(std::max<uint64_t>(1, ((max_nodes + 31) / 32 + kBlock - 1) / kBlock));
+    L.union_grid = b->inst[0]->max_grid;
     *out = reinterpret_cast<hb_batch *>(b.release());
   });
 }

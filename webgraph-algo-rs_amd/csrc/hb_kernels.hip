@@ -70,6 +70,13 @@ __device__ __forceinline__ uint32_t group_sum_u32(uint32_t v) {
   return v;
 }
 
+template <int G>
+__device__ __forceinline__ uint32_t group_or_u32(uint32_t v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, G);
+  return v;
+}
+
 // Per-register maximum of one slab: x = max(x, y) over 32 registers held as
 // R bit planes.  gt marks the registers where x > y and eq those still tied,
 // decided from the top plane down; d = x ^ y serves the tie update and the
@@ -202,7 +209,7 @@ __device__ __forceinline__ void store_part(uint32_t *dst, const uint32_t *src) {
 
 struct Acc {
   int64_t nf_int = 0, nf_hi = 0, nf_lo = 0;
-  uint64_t visited = 0, scanned = 0, tscanned = 0;
+  uint64_t visited = 0, scanned = 0, tscanned = 0, gathered = 0, changed = 0;
   uint32_t checked = 0, copied = 0, modified = 0;
 
   // Every step is exact except the final rint (|error| <= 2^-65).
@@ -239,17 +246,17 @@ __device__ __forceinline__ void acc_add(void *base, uint32_t mode, uint32_t v, d
 
 // Wavefront sum of the per-lane accumulators, added to *st by lane 0.  Every
 // lane of the wavefront must call it.
-__device__ void flush_stats(const Acc &a, Stats *st) {
-  uint64_t v[9] = {uint64_t(a.nf_int), uint64_t(a.nf_hi), uint64_t(a.nf_lo), a.visited, a.scanned,
-                   a.tscanned, a.checked, a.copied, a.modified};
+__device__ __forceinline__ void flush_stats(const Acc &a, Stats *st) {
+  uint64_t v[11] = {uint64_t(a.nf_int), uint64_t(a.nf_hi), uint64_t(a.nf_lo), a.visited, a.scanned,
+                    a.tscanned, a.checked, a.copied, a.modified, a.gathered, a.changed};
 #pragma unroll
-  for (int k = 0; k < 9; k++)
+  for (int k = 0; k < 11; k++)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
   if (__lane_id() == 0) {
-    unsigned long long *p = reinterpret_cast<unsigned long long *>(st);
+    unsigned long long *p = reinterpret_cast<unsigned long long *>(st);  // Stats field order
 #pragma unroll
-    for (int k = 0; k < 9; k++)
+    for (int k = 0; k < 11; k++)
       if (v[k]) atomicAdd(p + k, (unsigned long long)v[k]);
   }
 }
@@ -304,15 +311,25 @@ __host__ __device__ constexpr int slice_width() {
 // found in the slices of v's row that start at first, first + stride, ...
 // (hyperball_impl.rs:1037-1049).  `prev` / `cur` are loaded from the node's
 // own counter on the first hit.  The ids of the modified successors of a
-// slice are packed, in ballot order, into the lane group's LDS slot `ids`
-// (slice_width<G>() words); they are then merged U at a time with no
-// per-slot predication: a short last batch repeats its first id, and merging
-// a counter twice changes nothing.
-template <int R, int G, int SPL>
-__device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v, int gl,
+// slice and their change masks are packed, in ballot order, into the lane
+// group's LDS slot (`ids`: slice_width<G>() ids, then as many masks); they
+// are then merged U at a time with no per-slot predication: a short last
+// batch repeats its first id, and merging a counter twice changes nothing.
+//
+// Delta gathering: a lane reads only the slabs of a successor whose bit is
+// set in the successor's change mask, and merges zeros for the others.  This
+// is exact.  For t >= 1 every counter already dominates its successors'
+// counters of iteration t - 1: iteration 0 checks all nodes; a node checked
+// in iteration t - 1 merged them; a node not checked had no successor
+// modified in iteration t - 2, so it still dominates their unchanged
+// counters.  So only registers that changed in iteration t - 1 can raise it.
+// In iteration 0 the mask holds the slabs with a non-zero register.
+template <int R, int G, int SPL, bool DELTA>
+__device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int gl,
                                                  uint64_t first, uint64_t end, uint64_t stride,
                                                  uint32_t *prev, uint32_t *cur, bool &loaded,
-                                                 uint32_t &nvisit, uint32_t *ids) {
+                                                 uint32_t &nvisit, uint32_t &acc_gathered,
+                                                 uint32_t *ids) {
   constexpr int D = R * SPL;         // dwords per lane
   constexpr int CD = G * D;          // dwords per counter
   constexpr int K = ids_per_lane<G>();
@@ -323,20 +340,49 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
 #endif
   const uint32_t *own = a.curr + uint64_t(v) * CD + gl * D;
   const uint32_t *lane_base = a.curr + gl * D;
+  uint32_t *masks = ids + slice_width<G>();
   const uint64_t below = (uint64_t(1) << gl) - 1;
+  // this lane's slabs in change-mask bits
+  uint32_t lane_bit[SPL];
+#pragma unroll
+  for (int q = 0; q < SPL; q++) lane_bit[q] = 1u << ((uint32_t(gl) * SPL + q) >> a.chg_shift);
+  // with every node checked most successors are modified: read the masks
+  // together with the modified bits instead of after them
+  const bool speculate = a.flags & F_CHECK_ALL;
   for (uint64_t base = first; base < end; base += stride) {
-    uint32_t s[K];
+    uint32_t s[K], mk[K];
+    bool f[K];
 #pragma unroll
     for (int j = 0; j < K; j++) {
       const uint64_t idx = base + uint64_t(j) * G + gl;
       s[j] = idx < end ? a.succ[idx] : v;
     }
+    if (!DELTA) {
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        f[j] = (s[j] != v) && test_bit(a.curr_mod, s[j]);
+        mk[j] = ~0u;
+      }
+    } else if (speculate) {
+#pragma unroll
+      for (int j = 0; j < K; j++) mk[j] = s[j] != v ? a.chg_cur[s[j]] : 0u;
+#pragma unroll
+      for (int j = 0; j < K; j++) f[j] = (s[j] != v) && test_bit(a.curr_mod, s[j]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < K; j++) f[j] = (s[j] != v) && test_bit(a.curr_mod, s[j]);
+#pragma unroll
+      for (int j = 0; j < K; j++) mk[j] = f[j] ? a.chg_cur[s[j]] : 0u;
+    }
     uint32_t cnt = 0;
 #pragma unroll
     for (int j = 0; j < K; j++) {
-      const bool f = (s[j] != v) && test_bit(a.curr_mod, s[j]);
-      const uint64_t mask = group_ballot<G>(f);
-      if (f) ids[cnt + __popcll(mask & below)] = s[j];
+      const uint64_t mask = group_ballot<G>(f[j]);
+      if (f[j]) {
+        const uint32_t pos = cnt + __popcll(mask & below);
+        ids[pos] = s[j];
+        if (DELTA) masks[pos] = mk[j];
+      }
       cnt += __popcll(mask);
     }
     if (!cnt) continue;
@@ -352,8 +398,26 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
       uint32_t y[U][D];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const uint32_t t = ids[k + u < cnt ? k + u : k];
-        load_part<D>(y[u], lane_base + uint64_t(t) * CD);
+        const bool real = k + u < cnt;
+        const uint32_t slot = real ? k + u : k;
+        const uint32_t t = ids[slot];
+        const uint32_t *src = lane_base + uint64_t(t) * CD;
+        if (!DELTA) {
+          load_part<D>(y[u], src);
+          acc_gathered += real ? uint32_t(SPL) : 0u;
+          continue;
+        }
+        const uint32_t mt = masks[slot];
+#pragma unroll
+        for (int q = 0; q < SPL; q++) {
+          if (mt & lane_bit[q]) {
+            load_part<R>(y[u] + q * R, src + q * R);
+            acc_gathered += real ? 1u : 0u;
+          } else {
+#pragma unroll
+            for (int b = 0; b < R; b++) y[u][q * R + b] = 0u;
+          }
+        }
       }
 #pragma unroll
       for (int u = 0; u < U; u++)
@@ -362,6 +426,20 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
     }
     lane_sync<G>();
   }
+}
+
+template <int R, int G, int SPL>
+__device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v, int gl,
+                                                 uint64_t first, uint64_t end, uint64_t stride,
+                                                 uint32_t *prev, uint32_t *cur, bool &loaded,
+                                                 uint32_t &nvisit, uint32_t &acc_gathered,
+                                                 uint32_t *ids) {
+  if (a.flags & F_DELTA)
+    merge_slices<R, G, SPL, true>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                  acc_gathered, ids);
+  else
+    merge_slices<R, G, SPL, false>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                   acc_gathered, ids);
 }
 
 // Everything after the successor loop (hyperball_impl.rs:1051-1144): changed
@@ -375,11 +453,23 @@ __device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int 
   constexpr int D = R * SPL, CD = G * D;
   const bool systolic = a.flags & F_SYSTOLIC;
   bool cm = false;
+  uint32_t chg_bits = 0;  // this lane's changed slabs, in change-mask bits
   if (loaded) {
-    uint32_t d = 0;
 #pragma unroll
-    for (int k = 0; k < D; k++) d |= cur[k] ^ prev[k];
-    cm = group_ballot<G>(d != 0) != 0;
+    for (int q = 0; q < SPL; q++) {
+      uint32_t d = 0;
+#pragma unroll
+      for (int b = 0; b < R; b++) d |= cur[q * R + b] ^ prev[q * R + b];
+      if (d) chg_bits |= 1u << ((uint32_t(gl) * SPL + q) >> a.chg_shift);
+    }
+    cm = group_ballot<G>(chg_bits != 0) != 0;
+  }
+  if (cm) {
+    chg_bits = group_or_u32<G>(chg_bits);
+    if (gl == 0) {
+      a.chg_next[v] = chg_bits;
+      acc_out.changed += __popc(chg_bits);
+    }
   }
   double post = 0.0;
   if (cm) post = group_count<R, G, SPL>(cur, a.alpha_m_m, a.m, a.lc_table);
@@ -483,18 +573,19 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, const NodeMeta 
 
   uint32_t prev[D], cur[D];
   bool loaded = false;
-  uint32_t nvisit = 0;
+  uint32_t nvisit = 0, gathered = 0;
   if (check)
     merge_successors<R, G, SPL>(a, v, gl, beg, end, slice_width<G>(), prev, cur, loaded, nvisit,
-                                ids);
+                                gathered, ids);
+  acc_out.gathered += gathered;
   finish_node<R, G, SPL>(a, v, gl, check, was_mod, loaded, prev, cur, nvisit,
                          check ? end - beg : 0, acc_out);
 }
 
 
 // `wave_ids` and `meta` are the calling wavefront's own LDS regions, at least
-// 64 * ids_per_lane<G>() words and chunk_positions<G>() entries; its lane
-// groups take consecutive id slots.  The regions are per wavefront, not per
+// 2 * 64 * ids_per_lane<G>() words and chunk_positions<G>() entries; its lane
+// groups take consecutive id-and-mask slots.  The regions are per wavefront, not per
 // shape: in the batched kernel the wavefronts of one workgroup may run
 // different shapes at the same time.
 template <int R, int G, int SPL>
@@ -502,7 +593,7 @@ __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, ui
                                           uint32_t *wave_ids, NodeMeta *meta, Acc &acc) {
   constexpr int NPW = 64 / G;
   const int g = __lane_id() / G, gl = __lane_id() % G;
-  uint32_t *ids = wave_ids + g * slice_width<G>();
+  uint32_t *ids = wave_ids + g * 2 * slice_width<G>();
   stage_chunk<G>(a, chunk, count, meta);
 #pragma unroll 1
   for (int round = 0; round < kRoundsPerChunk; round++) {
@@ -517,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
   const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
   const uint64_t chunks = (count + chunk_positions<G>() - 1) / chunk_positions<G>();
-  constexpr int kWaveIds = 64 * ids_per_lane<G>();
+  constexpr int kWaveIds = 2 * 64 * ids_per_lane<G>();
   constexpr int kWaveMeta = int(chunk_positions<G>());
   __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
   __shared__ NodeMeta s_meta[(kBlock / 64) * kWaveMeta];
@@ -538,9 +629,13 @@ __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
 // kBlock/G lane groups take interleaved G-wide slices of the row, keep
 // private maxima, and fold them in LDS; lane group 0 finishes the node.  Every
 // thread of the block calls it with the same v.  s_acc holds kBlock * D words.
-template <int R, int G, int SPL>
-__device__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc, uint32_t *s_ids,
-                           uint32_t *s_visit, uint32_t *s_any, Acc &acc) {
+// DELTA_OK: whether the node may use delta gathering (when the iteration's
+// flags ask for it); the batched kernel keeps it off to stay within two
+// wavefronts per SIMD.
+template <int R, int G, int SPL, bool DELTA_OK = true>
+__device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc,
+                                           uint32_t *s_ids, uint32_t *s_visit, uint32_t *s_any,
+                                           Acc &acc) {
   constexpr int D = R * SPL, NG = kBlock / G, CD = G * D;
   const int g = threadIdx.x / G, gl = threadIdx.x % G;
   const bool in_list = !(a.flags & F_LIST) || test_bit(a.must, v) || test_bit(a.curr_mod, v);
@@ -550,11 +645,19 @@ __device__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc, uint
   const uint64_t beg = a.off[v], end = a.off[v + 1];
   uint32_t prev[D], cur[D];
   bool loaded = false;
-  uint32_t nvisit = 0;
-  if (check)
-    merge_successors<R, G, SPL>(a, v, gl, beg + uint64_t(g) * slice_width<G>(), end,
-                                uint64_t(NG) * slice_width<G>(), prev, cur, loaded, nvisit,
-                                s_ids + g * slice_width<G>());
+  uint32_t nvisit = 0, gathered = 0;
+  if (check) {
+    const uint64_t first = beg + uint64_t(g) * slice_width<G>();
+    const uint64_t stride = uint64_t(NG) * slice_width<G>();
+    uint32_t *slot = s_ids + g * 2 * slice_width<G>();
+    if (DELTA_OK && (a.flags & F_DELTA))
+      merge_slices<R, G, SPL, true>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                    gathered, slot);
+    else
+      merge_slices<R, G, SPL, false>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                     gathered, slot);
+  }
+  acc.gathered += gathered;
   if (threadIdx.x == 0) *s_any = 0;
   __syncthreads();
   if (loaded && gl == 0) atomicOr(s_any, 1u);
@@ -598,7 +701,7 @@ __device__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc, uint
 template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
   __shared__ uint32_t s_acc[kBlock * R * SPL];
-  __shared__ uint32_t s_ids[kBlock * ids_per_lane<G>()];
+  __shared__ uint32_t s_ids[2 * kBlock * ids_per_lane<G>()];
   __shared__ uint32_t s_visit[kBlock];
   __shared__ uint32_t s_any;
   __shared__ uint64_t s_h;
@@ -639,11 +742,13 @@ __global__ __launch_bounds__(kBlock) void seed_kernel(SeedArgs a) {
     uint32_t *c = a.counters + v * a.cdw;
     const uint64_t items = a.weights ? a.weights[v] : 1;
     const uint64_t first = a.weights ? a.weight_prefix[v] : 0;
+    uint32_t chg = 0;
     for (uint64_t k = 0; k < items; k++) {
       const uint64_t item = a.weights ? weight_item(a.weight_seed, first + k) : v;
       const uint64_t x = a.hash == 1 ? jenkins(item, a.seed) : siphash13(item);
       const uint32_t j = uint32_t(x & ((uint64_t(1) << a.log2m) - 1));
       const uint32_t val = uint32_t(__builtin_ctzll((x >> a.log2m) | a.sentinel)) + 1;
+      chg |= 1u << ((j >> 5) >> a.chg_shift);
       uint32_t *slab = c + (j >> 5) * a.r;
       const uint32_t q = j & 31;
       uint32_t cur = 0;
@@ -652,6 +757,7 @@ __global__ __launch_bounds__(kBlock) void seed_kernel(SeedArgs a) {
         for (uint32_t b = 0; b < a.r; b++)
           slab[b] = (slab[b] & ~(1u << q)) | (((val >> b) & 1u) << q);
     }
+    a.chg[v] = chg;
   }
 }
 
@@ -791,32 +897,6 @@ __global__ void to_reference_kernel(const uint32_t *cnt, uint64_t node_begin, ui
   X(0, 5, HB_G_6, HB_SPL_6) X(1, 5, HB_G_8, HB_SPL_8) X(2, 5, HB_G_10, HB_SPL_10)             \
   X(3, 6, HB_G_6, HB_SPL_6) X(4, 6, HB_G_8, HB_SPL_8) X(5, 6, HB_G_10, HB_SPL_10)
 
-// LDS needs of the batched kernels: the widest per-lane counter part (the
-// heavy kernel's fold buffer holds kBlock of them) and the largest per-wave
-// successor-id region, over all batched shapes.
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
-constexpr int batch_max_d() {
-  int d = 0;
-#define X(ID, RR, GG, SS) d = cmax(d, RR * SS);
-  HB_BATCH_SHAPES(X)
-#undef X
-  return d;
-}
-constexpr int batch_wave_meta() {
-  int w = 0;
-#define X(ID, RR, GG, SS) w = cmax(w, int(chunk_positions<GG>()));
-  HB_BATCH_SHAPES(X)
-#undef X
-  return w;
-}
-constexpr int batch_wave_ids() {
-  int w = 0;
-#define X(ID, RR, GG, SS) w = cmax(w, 64 * ids_per_lane<GG>());
-  HB_BATCH_SHAPES(X)
-#undef X
-  return w;
-}
-
 __device__ __forceinline__ uint64_t shape_chunk_positions(uint32_t shape) {
   switch (shape) {
 #define X(ID, RR, GG, SS) \
@@ -827,16 +907,18 @@ __device__ __forceinline__ uint64_t shape_chunk_positions(uint32_t shape) {
   return 1;
 }
 
-// First instance whose [base, end) range of work items contains x.
-__device__ __forceinline__ int find_instance(const BatchDesc *d, int k, uint64_t x, bool heavy) {
-  int lo = 0, hi = k - 1;
+// Instance whose [base, end) range of work items contains x, among the
+// instances of one shape (`inst`, in range order).
+__device__ __forceinline__ uint32_t find_in_shape(const BatchDesc *d, const uint32_t *inst, int cnt,
+                                                  uint64_t x, bool heavy) {
+  int lo = 0, hi = cnt - 1;
   while (lo < hi) {
     const int mid = (lo + hi) / 2;
-    const uint64_t e = heavy ? d[mid].heavy_end : d[mid].chunk_end;
+    const uint64_t e = heavy ? d[inst[mid]].heavy_end : d[inst[mid]].chunk_end;
     if (e > x) hi = mid;
     else lo = mid + 1;
   }
-  return lo;
+  return inst[lo];
 }
 
 // grid (x, instance): clears the next modified bitmap (and next must-be-checked
@@ -848,7 +930,7 @@ __global__ __launch_bounds__(kBlock) void batch_prep_kernel(const BatchDesc *d, 
     uint32_t *p = reinterpret_cast<uint32_t *>(stats + blockIdx.y);
     for (uint32_t i = threadIdx.x; i < sizeof(Stats) / 4; i += blockDim.x) p[i] = 0;
     if (blockIdx.y == 0)
-      for (uint32_t i = threadIdx.x; i < kWorkCounters * kWorkStride; i += blockDim.x) work[i] = 0;
+      for (uint32_t i = threadIdx.x; i < kBatchWorkWords; i += blockDim.x) work[i] = 0;
   }
   if (!x.active) return;
   for (uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < x.words;
@@ -893,88 +975,88 @@ __global__ __launch_bounds__(kBlock) void batch_compact_emit_kernel(const BatchD
   }
 }
 
-// One thread: positions of every active instance, chunk prefix sums.
-__global__ void batch_plan_kernel(BatchDesc *d, int k, Stats *stats, uint64_t *total_chunks) {
+// One thread: the positions of every active instance and, shape by shape,
+// the prefix sums of their chunks (each shape has its own chunk space).
+__global__ void batch_plan_kernel(BatchDesc *d, const uint32_t *shape_inst,
+                                  const uint32_t *shape_off, Stats *stats, uint64_t *total_chunks) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint64_t acc = 0;
-  for (int i = 0; i < k; i++) {
-    uint64_t count = 0;
-    if (d[i].active) count = d[i].list_mode ? uint64_t(*d[i].u.list_len) : d[i].n;
-    stats[i].list_len = count;
-    const uint64_t ch = shape_chunk_positions(d[i].shape);
-    d[i].chunk_base = acc;
-    acc += (count + ch - 1) / ch;
-    d[i].chunk_end = acc;
+  for (int sh = 0; sh < kBatchShapes; sh++) {
+    uint64_t acc = 0;
+    for (uint32_t j = shape_off[sh]; j < shape_off[sh + 1]; j++) {
+      BatchDesc &x = d[shape_inst[j]];
+      uint64_t count = 0;
+      if (x.active) count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
+      stats[shape_inst[j]].list_len = count;
+      const uint64_t ch = shape_chunk_positions(x.shape);
+      x.chunk_base = acc;
+      acc += (count + ch - 1) / ch;
+      x.chunk_end = acc;
+    }
+    total_chunks[sh] = acc;
   }
-  *total_chunks = acc;
 }
 
-__global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d, int k,
-                                                             const uint64_t *total_chunks,
+// The union step of one batched shape: per-XCD queues over the shape's chunk
+// space; a wavefront finds the instance of its chunk and runs it.
+template <int R, int G, int SPL>
+__global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d, const uint32_t *inst,
+                                                             int cnt, const uint64_t *total_chunks,
                                                              uint32_t *work) {
-  constexpr int kWaveIds = batch_wave_ids();
-  constexpr int kWaveMeta = batch_wave_meta();
+  constexpr int kWaveIds = 2 * 64 * ids_per_lane<G>();
+  constexpr int kWaveMeta = int(chunk_positions<G>());
   __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
   __shared__ NodeMeta s_meta[(kBlock / 64) * kWaveMeta];
   uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
   NodeMeta *meta = s_meta + (threadIdx.x / 64) * kWaveMeta;
   const uint64_t chunks = *total_chunks;
   Acc acc;
-  int current = -1;
+  int64_t current = -1;
   for (;;) {
     uint64_t c = 0;
     if (__lane_id() == 0) c = grab_chunk(work, chunks);
     c = __shfl(c, 0);
     if (c == UINT64_MAX) break;
-    const int i = find_instance(d, k, c, false);
-    if (i != current) {
+    const uint32_t i = find_in_shape(d, inst, cnt, c, false);
+    if (int64_t(i) != current) {
       if (current >= 0) flush_stats(acc, d[current].u.stats);
       acc = Acc();
       current = i;
     }
     const BatchDesc &x = d[i];
     const uint64_t count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
-    switch (x.shape) {
-#define X(ID, RR, GG, SS) \
-  case ID: run_chunk<RR, GG, SS>(x.u, c - x.chunk_base, count, wave_ids, meta, acc); break;
-      HB_BATCH_SHAPES(X)
-#undef X
-    }
+    run_chunk<R, G, SPL>(x.u, c - x.chunk_base, count, wave_ids, meta, acc);
   }
   if (current >= 0) flush_stats(acc, d[current].u.stats);
 }
 
-__global__ __launch_bounds__(kBlock) void batch_heavy_kernel(const BatchDesc *d, int k,
-                                                             uint64_t total_heavy,
+// The heavy rows of one batched shape, one workgroup per row.
+template <int R, int G, int SPL>
+__global__ __launch_bounds__(kBlock) void batch_heavy_kernel(const BatchDesc *d, const uint32_t *inst,
+                                                             int cnt, uint64_t total_heavy,
                                                              uint32_t *work) {
-  __shared__ uint32_t s_acc[kBlock * batch_max_d()];
-  __shared__ uint32_t s_ids[(kBlock / 64) * batch_wave_ids()];
-  __shared__ uint32_t s_visit[kBlock];  // (one shape per workgroup at a time)
+  __shared__ uint32_t s_acc[kBlock * R * SPL];
+  __shared__ uint32_t s_ids[2 * kBlock * ids_per_lane<G>()];
+  __shared__ uint32_t s_visit[kBlock];
   __shared__ uint32_t s_any;
   __shared__ uint64_t s_h;
   Acc acc;
-  int current = -1;
+  int64_t current = -1;
   for (;;) {
     if (threadIdx.x == 0) s_h = atomicAdd(work + 8 * kWorkStride, 1u);
     __syncthreads();
     const uint64_t h = s_h;
     __syncthreads();
     if (h >= total_heavy) break;
-    const int i = find_instance(d, k, h, true);
+    const uint32_t i = find_in_shape(d, inst, cnt, h, true);
     const BatchDesc &x = d[i];
     if (!x.active) continue;
-    if (i != current) {
+    if (int64_t(i) != current) {
       if (current >= 0) flush_stats(acc, d[current].u.stats);
       acc = Acc();
       current = i;
     }
-    const uint32_t v = x.u.heavy_list[h - x.heavy_base];
-    switch (x.shape) {
-#define X(ID, RR, GG, SS) \
-  case ID: heavy_node<RR, GG, SS>(x.u, v, s_acc, s_ids, s_visit, &s_any, acc); break;
-      HB_BATCH_SHAPES(X)
-#undef X
-    }
+    heavy_node<R, G, SPL>(x.u, x.u.heavy_list[h - x.heavy_base], s_acc, s_ids, s_visit, &s_any,
+                          acc);
   }
   if (current >= 0) flush_stats(acc, d[current].u.stats);
 }
@@ -1035,17 +1117,31 @@ int batch_shape(int r, int log2m) {
   return (r - 5) * 3 + (log2m - 6) / 2;
 }
 
-hipError_t launch_batch_iteration(BatchDesc *d, int k, Stats *stats, uint32_t *work,
-                                  uint64_t *total_chunks, uint32_t prep_blocks,
-                                  uint32_t compact_blocks, uint32_t union_grid,
-                                  uint32_t heavy_grid, uint64_t total_heavy, hipStream_t s) {
-  batch_prep_kernel<<<dim3(prep_blocks, k), kBlock, 0, s>>>(d, stats, work);
-  batch_compact_count_kernel<<<dim3(compact_blocks, k), kBlock, 0, s>>>(d);
-  batch_compact_scan_kernel<<<k, 1024, 0, s>>>(d);
-  batch_compact_emit_kernel<<<dim3(compact_blocks, k), kBlock, 0, s>>>(d);
-  batch_plan_kernel<<<1, 64, 0, s>>>(d, k, stats, total_chunks);
-  if (heavy_grid) batch_heavy_kernel<<<heavy_grid, kBlock, 0, s>>>(d, k, total_heavy, work);
-  batch_union_kernel<<<union_grid, kBlock, 0, s>>>(d, k, total_chunks, work);
+hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
+  batch_prep_kernel<<<dim3(L.prep_blocks, L.k), kBlock, 0, s>>>(L.d, L.stats, L.work);
+  batch_compact_count_kernel<<<dim3(L.compact_blocks, L.k), kBlock, 0, s>>>(L.d);
+  batch_compact_scan_kernel<<<L.k, 1024, 0, s>>>(L.d);
+  batch_compact_emit_kernel<<<dim3(L.compact_blocks, L.k), kBlock, 0, s>>>(L.d);
+  batch_plan_kernel<<<1, 64, 0, s>>>(L.d, L.shape_inst, L.shape_off, L.stats, L.total_chunks);
+  for (int sh = 0; sh < kBatchShapes; sh++) {
+    if (!L.count[sh]) continue;
+    const uint32_t *inst = L.shape_inst + L.first[sh];
+    const int cnt = int(L.count[sh]);
+    uint32_t *work = L.work + size_t(sh) * kWorkCounters * kWorkStride;
+    switch (sh) {
+#define X(ID, RR, GG, SS)                                                                      \
+  case ID:                                                                                     \
+    if (L.heavy_grid[sh])                                                                      \
+      batch_heavy_kernel<RR, GG, SS><<<L.heavy_grid[sh], kBlock, 0, s>>>(L.d, inst, cnt,      \
+                                                                          L.total_heavy[sh],   \
+                                                                          work);               \
+    batch_union_kernel<RR, GG, SS><<<L.union_grid, kBlock, 0, s>>>(L.d, inst, cnt,            \
+                                                                    L.total_chunks + sh, work); \
+    break;
+      HB_BATCH_SHAPES(X)
+#undef X
+    }
+  }
   return hipGetLastError();
 }
 

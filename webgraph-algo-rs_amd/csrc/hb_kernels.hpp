@@ -25,6 +25,7 @@ enum UnionFlags : uint32_t {
   F_CHECK_ALL = 2u,   // every processed node enumerates its successors
   F_CENTRALITY = 4u,  // update the per-node accumulators
   F_LIST = 8u,        // nodes come from a compacted list whose length lives on the device
+  F_DELTA = 16u,      // gather only the changed slabs of successors (change masks)
 };
 
 enum AccMode : uint32_t { ACC_F64 = 0, ACC_F32_KAHAN = 1, ACC_F32 = 2 };
@@ -38,6 +39,8 @@ enum AccMode : uint32_t { ACC_F64 = 0, ACC_F32_KAHAN = 1, ACC_F32 = 2 };
 struct Stats {
   int64_t nf_int, nf_hi, nf_lo;
   uint64_t visited, scanned, tscanned, checked, copied, modified;
+  uint64_t gathered;  // successor slabs read
+  uint64_t changed;   // set change-mask bits over the modified nodes
   uint64_t list_len;
 };
 
@@ -86,6 +89,13 @@ struct UnionArgs {
   uint64_t heavy_degree;         // rows longer than this go to heavy_kernel
   const uint32_t *heavy_list;    // the nodes with such rows
   uint64_t heavy_count;
+  // Change masks: bit (slab >> chg_shift) of chg_cur[s] is set when a
+  // register of that slab of node s changed in the last iteration (or, in
+  // iteration 0, is not zero).  Read for modified successors, written for
+  // modified nodes.
+  const uint32_t *chg_cur;
+  uint32_t *chg_next;
+  uint32_t chg_shift;
 };
 
 struct SeedArgs {
@@ -99,6 +109,8 @@ struct SeedArgs {
   const uint64_t *weights;       // nullable
   const uint64_t *weight_prefix; // exclusive prefix sums of weights
   uint64_t weight_seed;
+  uint32_t *chg;        // per node: the slabs holding a non-zero register
+  uint32_t chg_shift;
 };
 
 // ---- batched execution --------------------------------------------------
@@ -115,17 +127,35 @@ struct BatchDesc {
   uint64_t words;  // 32-bit words of each bitmap
   uint64_t n;
   uint32_t *bsum;  // compaction scratch
-  uint64_t chunk_base, chunk_end;  // written on the device by the plan step
-  uint64_t heavy_base, heavy_end;  // prefix of the instances' heavy rows
+  uint64_t chunk_base, chunk_end;  // in the shape's chunk space, written by the plan step
+  uint64_t heavy_base, heavy_end;  // in the shape's heavy-row space
 };
 
-// Launches one batched iteration (all instances) on s: clear the next
-// bitmaps and statistics, compact the worklists of list-mode instances, plan
-// the chunk space, then the heavy-row and union kernels.
-hipError_t launch_batch_iteration(BatchDesc *d, int k, Stats *stats, uint32_t *work,
-                                  uint64_t *total_chunks, uint32_t prep_blocks,
-                                  uint32_t compact_blocks, uint32_t union_grid,
-                                  uint32_t heavy_grid, uint64_t total_heavy, hipStream_t s);
+// Queue counters of a batch: one region of kWorkCounters per shape.
+constexpr int kBatchWorkWords = kBatchShapes * kWorkCounters * kWorkStride;
+
+// One batched iteration.  Instances are grouped by shape (`shape_inst`,
+// `shape_off` on the device; `first`/`count` on the host): each shape has its
+// own chunk space, heavy-row space and queue counters, and its own heavy and
+// union kernels, compiled for that shape alone.
+struct BatchLaunch {
+  BatchDesc *d;
+  int k;
+  Stats *stats;
+  uint32_t *work;           // kBatchWorkWords
+  uint64_t *total_chunks;   // kBatchShapes, written by the plan step
+  const uint32_t *shape_inst;
+  const uint32_t *shape_off;  // kBatchShapes + 1
+  uint32_t first[kBatchShapes], count[kBatchShapes];
+  uint32_t prep_blocks, compact_blocks, union_grid;
+  uint32_t heavy_grid[kBatchShapes];
+  uint64_t total_heavy[kBatchShapes];
+};
+
+// Launches one batched iteration on s: clear the next bitmaps and statistics,
+// compact the worklists of list-mode instances, plan the chunk spaces, then
+// per shape the heavy-row and union kernels.
+hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s);
 
 // Chooses the lane group of a (r, log2m) shape: lanes per node and slabs per
 // lane.  Returns false for shapes without a kernel.

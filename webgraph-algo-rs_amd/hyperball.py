@@ -73,7 +73,8 @@ class IterationStats(ctypes.Structure):
                 ("checked", ctypes.c_uint64), ("visited_arcs", ctypes.c_uint64),
                 ("scanned_arcs", ctypes.c_uint64), ("transpose_arcs", ctypes.c_uint64),
                 ("copied", ctypes.c_uint64), ("modified", ctypes.c_uint64), ("nf", ctypes.c_double),
-                ("union_ms", ctypes.c_double), ("iteration_ms", ctypes.c_double)]
+                ("union_ms", ctypes.c_double), ("iteration_ms", ctypes.c_double),
+                ("gathered_slabs", ctypes.c_uint64), ("delta_gathering", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -130,6 +131,7 @@ _SIGNATURES = [
     ("hb_batch_run_until_done", _I, [_P]),
     ("hb_batch_set_graph", _I, [_P, _I]),
     ("hb_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
+    ("hb_set_delta_gathering", _I, [_P, _I]),
     ("hb_batch_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
     ("hb_batch_step_times", _I, [_P, _P, _SZ, _P]),
 ]
@@ -528,6 +530,10 @@ class HyperBall:
                                  ctypes.byref(chosen)))
         k = min(ln.value, cap)
         return chosen.value, [(int(th[i]), float(ms[i])) for i in range(k)]
+
+    def set_delta_gathering(self, mode: int) -> None:
+        """0: per iteration by the library's rule, 1: always, 2: never."""
+        _check(lib().hb_set_delta_gathering(self._h, int(mode)))
 
     def set_tuning(self, heavy_degree: int) -> None:
         """Rows longer than heavy_degree go to the workgroup-per-node kernel
