@@ -335,6 +335,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="instances in the batched leg (0 = skip)")
     ap.add_argument("--only-batch", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the uniform n=1e5 leg")
+    ap.add_argument("--heavy-threshold", type=int, default=None,
+                    help="fixed heavy-row threshold instead of the autotuner (for profiling)")
     ap.add_argument("--precision", action="store_true",
                     help="only the accumulator precision study (log2m 12 on the web graph)")
     ap.add_argument("--large", action="store_true",
@@ -383,10 +385,14 @@ def main():
           file=sys.stderr, flush=True)
     hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, args.log2m).register_bits(args.reg_bits)
           .jenkins(42).sum_of_distances(True).sum_of_inverse_distances(True).build())
-    chosen, table = hb.autotune(reps=2)
-    print("# autotune (heavy-row threshold, dense-iteration ms): " +
-          ", ".join(f"{'none' if t == 2**32 - 1 else t}: {ms:.2f}" for t, ms in table) +
-          f" -> {chosen}", file=sys.stderr, flush=True)
+    if args.heavy_threshold is None:
+        chosen, table = hb.autotune(reps=2)
+        print("# autotune (heavy-row threshold, dense-iteration ms): " +
+              ", ".join(f"{'none' if t == 2**32 - 1 else t}: {ms:.2f}" for t, ms in table) +
+              f" -> {chosen}", file=sys.stderr, flush=True)
+    else:
+        chosen = args.heavy_threshold
+        hb.set_tuning(chosen)
     hb.set_timing(True)
     p = hb.params()
     m, counter_bytes = p["num_registers"], p["words_per_counter"] * 8
