@@ -53,12 +53,7 @@ def algorithmic_bytes(s, n, counter_bytes, slab_bytes, outputs, delta_aware=True
             + 16 * s["modified"] * outputs + 4 * ((n + 7) // 8))
 
 
-def cpu_baseline(orc, hbm, args, log2m, reg_bits):
-    """The oracle's OpenMP restatement of the reference engine, on a bounded
-    sample of the same workload: the same generator at n = args.cpu_nodes."""
-    g = hbm.Graph.web(args.cpu_nodes, 2.1, 3, 10000, 1000, 0.7, 2)
-    gt = g.transpose()
-    threads = host_threads()
+def _oracle_rate(orc, g, gt, log2m, reg_bits, threads, budget):
     ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=log2m,
                               reg_bits=reg_bits, hash=orc.HASH_JENKINS, seed=42,
                               sum_of_distances=True, sum_of_inverse_distances=True, threads=threads)
@@ -69,12 +64,28 @@ def cpu_baseline(orc, hbm, args, log2m, reg_bits):
     while True:
         ref.iterate()
         work += ref.visited_arcs * m
-        if ref.modified == 0 or time.perf_counter() - t0 > args.cpu_budget:
+        if ref.modified == 0 or time.perf_counter() - t0 > budget:
             break
     dt = time.perf_counter() - t0
-    return {"value": work / dt, "unit": "arc-registers/s", "cores": threads, "kind": "port",
+    return work / dt, ref.iteration, dt
+
+
+def cpu_baseline(orc, hbm, args, log2m, reg_bits):
+    """The oracle's OpenMP restatement of the reference engine, on a bounded
+    sample of the same workload: the same generator at n = args.cpu_nodes
+    on the host's cores (up to 16), and at n / 5 on one core."""
+    threads = host_threads()
+    g = hbm.Graph.web(args.cpu_nodes, 2.1, 3, 10000, 1000, 0.7, 2)
+    gt = g.transpose()
+    rate, iters, dt = _oracle_rate(orc, g, gt, log2m, reg_bits, threads, args.cpu_budget)
+    g1 = hbm.Graph.web(max(1, args.cpu_nodes // 5), 2.1, 3, 10000, 1000, 0.7, 2)
+    g1t = g1.transpose()
+    rate1, iters1, dt1 = _oracle_rate(orc, g1, g1t, log2m, reg_bits, 1, args.cpu_budget / 2)
+    return {"value": rate, "unit": "arc-registers/s", "cores": threads, "kind": "port",
             "sample": f"web graph n={g.num_nodes} arcs={g.num_arcs} (same generator), log2m={log2m}, "
-                      f"r={reg_bits}, {ref.iteration} iterations in {dt:.1f}s"}
+                      f"r={reg_bits}, {iters} iterations in {dt:.1f}s",
+            "one_core": {"value": rate1, "sample": f"n={g1.num_nodes} arcs={g1.num_arcs}, "
+                                                   f"{iters1} iterations in {dt1:.1f}s"}}
 
 
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "union_step_web1e7_log2m10_r6_pmc.md")

@@ -225,6 +225,16 @@ hb_status hb_num_iterations(hb_instance *hb, uint64_t *out);
 hb_status hb_get_iteration_stats(hb_instance *hb, uint64_t iteration, hb_iteration_stats *out);
 /* Turns per-kernel HIP-event timing on or off (off by default). */
 hb_status hb_set_timing(hb_instance *hb, int enabled);
+/* Per-iteration progress (the reference's `pl: ConcurrentProgressLog`
+ * argument of run, hyperball_impl.rs:534-583): f(stats, ctx) is called on the
+ * calling thread after every iteration; NULL turns it off. */
+typedef void (*hb_progress_fn)(const hb_iteration_stats *stats, void *ctx);
+hb_status hb_set_progress(hb_instance *hb, hb_progress_fn f, void *ctx);
+/* Runs the instance's work on the caller's HIP stream (a hipStream_t of the
+ * instance's device), e.g. to order it with the caller's own kernels; NULL
+ * gives the instance a private stream again.  Waits for the current stream
+ * first.  Not for the instances of a batch. */
+hb_status hb_set_stream(hb_instance *hb, void *stream);
 /* Overrides the lane-group split of the union kernel (0 restores the
  * default); for the autotuner.  heavy_degree: rows with more successors than
  * this go to the workgroup-per-node kernel. */

@@ -288,3 +288,32 @@ def test_instance_with_device_transpose(hbm):
         for k in ("systolic", "local", "pre_local", "checked", "visited_arcs", "transpose_arcs",
                   "modified"):
             assert x[k] == y[k], k
+
+
+def test_progress_callback_and_caller_stream(hbm):
+    """hb_set_progress reports every iteration as it ends; hb_set_stream runs
+    the instance on a caller's stream with the same results, and None gives
+    it a private stream again.  The caller's stream comes from the HIP runtime
+    the library itself uses."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    g, gt = _web(hbm, 20000, seed=4)
+    hb = hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).jenkins(1).sum_of_distances(True).build()
+    seen = []
+    hb.set_progress(lambda st: seen.append(st))
+    hb.run_until_done()
+    assert [s["iteration"] for s in seen] == list(range(len(hb.iteration_stats())))
+    assert [s["modified"] for s in seen] == [s["modified"] for s in hb.iteration_stats()]
+    sd_private = hb.sum_of_distances()
+    hb.set_progress(None)
+    stream = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+    hb.set_stream(stream.value)
+    hb.run_until_done()
+    assert hip.hipStreamSynchronize(stream) == 0
+    assert np.array_equal(hb.sum_of_distances(), sd_private)
+    hb.set_stream(None)
+    assert hip.hipStreamDestroy(stream) == 0
+    hb.run_until_done()
+    assert np.array_equal(hb.sum_of_distances(), sd_private)
+    assert len(seen) == len(hb.iteration_stats())  # no calls after None

@@ -77,6 +77,8 @@ struct Instance {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = true;
+  hb_progress_fn progress = nullptr;  // called after every iteration
+  void *progress_ctx = nullptr;
   bool in_batch = false;  // owned by a batch: hb_destroy refuses it
   uint32_t cus = 256;
 
@@ -378,6 +380,7 @@ struct Instance {
     history.push_back(hs);
     modified = st.modified;
     iteration++;
+    if (progress) progress(&history.back(), progress_ctx);
   }
 
   void iterate() {
@@ -1108,6 +1111,32 @@ hb_status hb_get_iteration_stats(hb_instance *hb, uint64_t iteration, hb_iterati
     if (!out) fail(HB_E_INVALID_ARG, "null output");
     if (iteration >= I.history.size()) fail(HB_E_INVALID_ARG, "no such iteration");
     *out = I.history[iteration];
+  });
+}
+
+hb_status hb_set_progress(hb_instance *hb, hb_progress_fn f, void *ctx) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.progress = f;
+    I.progress_ctx = ctx;
+  });
+}
+
+hb_status hb_set_stream(hb_instance *hb, void *stream) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    if (I.in_batch) fail(HB_E_INVALID_ARG, "the instances of a batch share the batch's stream");
+    I.use_device();
+    HB_HIP(hipStreamSynchronize(I.stream));
+    if (I.own_stream) HB_HIP(hipStreamDestroy(I.stream));
+    I.stream = nullptr;
+    if (stream) {
+      I.stream = static_cast<hipStream_t>(stream);
+      I.own_stream = false;
+    } else {
+      HB_HIP(hipStreamCreateWithFlags(&I.stream, hipStreamNonBlocking));
+      I.own_stream = true;
+    }
   });
 }
 

@@ -81,6 +81,7 @@ class IterationStats(ctypes.Structure):
 
 
 DISCOUNT_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(IterationStats), ctypes.c_void_p)
 
 # (name, restype, argtypes) of every entry point of include/hyperball.h
 _P, _U64, _U32, _D, _I, _SZ = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double, ctypes.c_int, ctypes.c_size_t
@@ -132,6 +133,8 @@ _SIGNATURES = [
     ("hb_batch_set_graph", _I, [_P, _I]),
     ("hb_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
     ("hb_set_delta_gathering", _I, [_P, _I]),
+    ("hb_set_progress", _I, [_P, _P, _P]),
+    ("hb_set_stream", _I, [_P, _P]),
     ("hb_batch_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
     ("hb_batch_step_times", _I, [_P, _P, _SZ, _P]),
 ]
@@ -530,6 +533,23 @@ class HyperBall:
                                  ctypes.byref(chosen)))
         k = min(ln.value, cap)
         return chosen.value, [(int(th[i]), float(ms[i])) for i in range(k)]
+
+    def set_progress(self, f: Optional[Callable[[dict], None]]) -> None:
+        """f(stats dict) after every iteration (None: off)."""
+        if f is None:
+            self._progress = None
+            _check(lib().hb_set_progress(self._h, None, None))
+            return
+
+        def cb(stats_ptr, _ctx):
+            f(stats_ptr.contents.as_dict())
+        self._progress = PROGRESS_FN(cb)
+        _check(lib().hb_set_progress(self._h, ctypes.cast(self._progress, ctypes.c_void_p), None))
+
+    def set_stream(self, stream: Optional[int]) -> None:
+        """Run on the caller's HIP stream (an integer handle, such as
+        torch.cuda.Stream().cuda_stream); None gives back a private stream."""
+        _check(lib().hb_set_stream(self._h, ctypes.c_void_p(stream) if stream else None))
 
     def set_delta_gathering(self, mode: int) -> None:
         """0: per iteration by the library's rule, 1: always, 2: never."""
