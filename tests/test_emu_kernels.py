@@ -40,12 +40,15 @@ def test_emulated_batch_matches_single_instances(emu_exes, graph, heavy, tune):
 
 
 @pytest.mark.parametrize("log2m,r,n,dmax,seed,heavy,delta", [
-    (6, 5, 500, 400, 4, 64, 1),     # 2 slabs, G = 2, heavy rows, delta gathering forced
+    (6, 5, 500, 400, 4, 64, 1),     # 2 slabs, G = 2, heavy rows, delta gathering forced;
+                                    # 10-dword counters at a 16-dword stride
     (8, 6, 400, 300, 5, 64, 1),     # 8 slabs, G = 8
     (10, 6, 300, 250, 3, 0, 0),     # 32 slabs: one change-mask bit each; the default rule
     (10, 5, 300, 250, 8, 64, 2),    # no delta gathering
     (12, 5, 200, 150, 6, 0, 1),     # 128 slabs: four per change-mask bit
-    (5, 6, 300, 250, 7, 64, 1),     # one slab, G = 1
+    (5, 6, 300, 250, 7, 64, 1),     # one slab, G = 1 (6 dwords, stride 8)
+    (8, 5, 300, 250, 9, 64, 0),     # 40 dwords, stride 48
+    (7, 5, 300, 250, 10, 0, 1),     # 20 dwords, stride 32
 ])
 def test_emulated_engine_matches_oracle(emu_exes, log2m, r, n, dmax, seed, heavy, delta):
     """Every iteration (standard, systolic, local, pre-local) of the emulated

@@ -84,6 +84,7 @@ struct Instance {
 
   uint64_t n = 0, a = 0;
   uint32_t log2m = 0, m = 0, r = 0, W = 0, cdw = 0;
+  uint32_t cstride = 0;  // dwords between counters: counter_stride(cdw)
   uint64_t sentinel = 0;
   double amm = 0.0;
   uint32_t hash = 0;
@@ -216,11 +217,11 @@ struct Instance {
 
   void init() {
     use_device();
-    for (auto &c : cnt) fill_zero(c.p, size_t(n) * cdw * 4);
+    for (auto &c : cnt) fill_zero(c.p, size_t(n) * cstride * 4);
     SeedArgs s{};
     s.counters = cnt[0].p;
     s.n = n;
-    s.cdw = cdw;
+    s.cdw = cstride;
     s.r = r;
     s.log2m = log2m;
     s.sentinel = sentinel;
@@ -784,6 +785,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     I.r = r;
     I.W = uint32_t(m * r / 64);
     I.cdw = uint32_t(m * r / 32);
+    I.cstride = counter_stride(I.cdw);
     while ((m / 32) >> I.chg_shift > 32) I.chg_shift++;
     I.sentinel = uint64_t(1) << ((1u << r) - 2);
     I.amm = alpha_m_m(cfg->log2m);
@@ -838,7 +840,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
 
     I.bitmap_words = std::max<uint64_t>(2, ((n + 63) / 64) * 2);
     for (int k = 0; k < 2; k++) {
-      I.cnt[k].alloc(size_t(n) * I.cdw);
+      I.cnt[k].alloc(size_t(n) * I.cstride);
       I.mod[k].alloc(I.bitmap_words);
       I.must[k].alloc(I.bitmap_words);
       I.chg[k].alloc(n);

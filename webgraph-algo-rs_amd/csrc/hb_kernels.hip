@@ -338,7 +338,8 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
 #else
   constexpr int U = D <= 6 ? 8 : D <= 12 ? 4 : 2;  // successor loads in flight per lane
 #endif
-  const uint32_t *own = a.curr + uint64_t(v) * CD + gl * D;
+  constexpr uint32_t CS = counter_stride(CD);  // dwords between counters
+  const uint32_t *own = a.curr + uint64_t(v) * CS + gl * D;
   const uint32_t *lane_base = a.curr + gl * D;
   uint32_t *masks = ids + slice_width<G>();
   const uint64_t below = (uint64_t(1) << gl) - 1;
@@ -401,7 +402,7 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
         const bool real = k + u < cnt;
         const uint32_t slot = real ? k + u : k;
         const uint32_t t = ids[slot];
-        const uint32_t *src = lane_base + uint64_t(t) * CD;
+        const uint32_t *src = lane_base + uint64_t(t) * CS;
         if (!DELTA) {
           load_part<D>(y[u], src);
           acc_gathered += real ? uint32_t(SPL) : 0u;
@@ -512,8 +513,8 @@ __device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int 
   }
 
   if (cm || was_mod) {
-    if (!loaded) load_part<D>(cur, a.curr + uint64_t(v) * CD + gl * D);
-    store_part<D>(a.next + uint64_t(v) * CD + gl * D, cur);
+    if (!loaded) load_part<D>(cur, a.curr + uint64_t(v) * counter_stride(CD) + gl * D);
+    store_part<D>(a.next + uint64_t(v) * counter_stride(CD) + gl * D, cur);
   }
 }
 
@@ -685,7 +686,7 @@ __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint3
     uint32_t total_visit = 0;
     for (int k = 0; k < NG; k++) total_visit += s_visit[k];
     if (any) {
-      load_part<D>(prev, a.curr + uint64_t(v) * CD + gl * D);
+      load_part<D>(prev, a.curr + uint64_t(v) * counter_stride(CD) + gl * D);
 #pragma unroll
       for (int k = 0; k < D; k++) cur[k] = s_acc[gl * D + k];
       // the folded maxima never lose the node's own registers
@@ -727,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void count_all_kernel(const uint32_t *cnt, 
     const uint64_t i = batch * NPB + g;
     if (i < n) {
       uint32_t x[D];
-      load_part<D>(x, cnt + i * CD + gl * D);
+      load_part<D>(x, cnt + i * counter_stride(CD) + gl * D);
       const double c = group_count<R, G, SPL>(x, alpha_m_m, m, lc);
       if (gl == 0) out[i] = c;
     }
@@ -881,7 +882,7 @@ __global__ void to_reference_kernel(const uint32_t *cnt, uint64_t node_begin, ui
        i += uint64_t(gridDim.x) * blockDim.x) {
     const uint64_t v = node_begin + i / W;
     const uint32_t k = uint32_t(i % W);
-    const uint32_t *c = cnt + v * cdw;
+    const uint32_t *c = cnt + v * counter_stride(cdw);
     uint64_t word = 0;
     for (uint32_t p = 0; p < 64; p++) {
       const uint32_t g = 64 * k + p, j = g / r, b = g % r;

@@ -18,6 +18,18 @@
 namespace hb {
 
 constexpr int kBlock = 256;
+
+// Distance in dwords between consecutive counters of cdw = m*r/32 dwords.
+// Counters sit on whole 64-byte sectors of HBM: below 16 dwords the stride is
+// the next power of two (a counter never straddles a sector), from 16 dwords
+// on a multiple of 16.  Random counter reads then move whole sectors only;
+// unpadded 40- and 48-byte counters straddle two sectors most of the time.
+__host__ __device__ constexpr uint32_t counter_stride(uint32_t cdw) {
+  if (cdw >= 16) return (cdw + 15) / 16 * 16;
+  uint32_t s = 1;
+  while (s < cdw) s <<= 1;
+  return s;
+}
 constexpr int kMaxDiscount = 8;
 
 enum UnionFlags : uint32_t {
@@ -101,7 +113,7 @@ struct UnionArgs {
 struct SeedArgs {
   uint32_t *counters;  // zeroed, slab layout
   uint64_t n;
-  uint32_t cdw;        // dwords per counter
+  uint32_t cdw;        // dwords between counters (counter_stride)
   uint32_t r, log2m;
   uint64_t sentinel;
   uint32_t hash;
