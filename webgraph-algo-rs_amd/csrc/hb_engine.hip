@@ -747,7 +747,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     if (!union_shape(int(r), int(cfg->log2m), &lanes, &spl))
       fail(HB_E_INVALID_ARG, "no kernel for register width " + std::to_string(r) + " and log2m " +
                                  std::to_string(cfg->log2m) +
-                                 " (this build holds widths 5 and 6, log2m 5..12)");
+                                 " (this library holds widths 5 and 6, log2m 5..12)");
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
