@@ -107,3 +107,26 @@ def test_batch_autotune_keeps_results(hbm):
     batch.run_until_done()
     for b, hb in zip(builders, batch.instances):
         _compare(hbm, b, hb)
+
+
+def test_batch_degenerate_instances(hbm):
+    """Instances with one node, no arcs, self loops only, a path and a clique
+    run in one batch beside a normal graph, each equal to its own run."""
+    n = 40
+    graphs = [hbm.Graph.from_arcs(1, []),
+              hbm.Graph.from_arcs(10, []),
+              hbm.Graph.from_arcs(10, [(i, i) for i in range(10)]),
+              hbm.Graph.from_arcs(n, [(i, i + 1) for i in range(n - 1)]),
+              hbm.Graph.from_arcs(12, [(i, j) for i in range(12) for j in range(12)]),
+              hbm.Graph.from_arcs(n, [(i, (i + 1) % n) for i in range(n)]),
+              hbm.Graph.web(5000, 2.1, 1, 300, 100, 0.7, 3)]
+    builders = []
+    for i, g in enumerate(graphs):
+        gt = g.transpose() if i != 5 else None
+        builders.append(hbm.HyperBallBuilder.with_hyper_log_log(g, gt, (6, 8, 10)[i % 3])
+                        .register_bits(5 + i % 2).jenkins(i).sum_of_distances(True)
+                        .sum_of_inverse_distances(True))
+    batch = hbm.HyperBallBatch(builders)
+    batch.run_until_done()
+    for b, hb in zip(builders, batch.instances):
+        _compare(hbm, b, hb)
