@@ -349,6 +349,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="instances in the batched leg (0 = skip)")
     ap.add_argument("--only-batch", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the uniform n=1e5 leg")
+    ap.add_argument("--only-small", action="store_true", help="only the uniform n=1e5 leg")
     ap.add_argument("--heavy-threshold", type=int, default=None,
                     help="fixed heavy-row threshold instead of the autotuner (for profiling)")
     ap.add_argument("--precision", action="store_true",
@@ -384,6 +385,10 @@ def main():
 
     if args.only_batch:
         print(json.dumps({"batched": bench_batch(hbm, args)}), flush=True)
+        return
+    if args.only_small:
+        orc = None if args.no_cpu_baseline else _load("hb_oracle_py", os.path.join(REPO, "oracle", "oracle.py"))
+        print(json.dumps({"uniform_1e5": small_uniform(hbm, orc, args)}), flush=True)
         return
     if args.precision:
         print(json.dumps({"precision": bench_precision(hbm, args)}), flush=True)

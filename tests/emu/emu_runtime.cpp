@@ -272,6 +272,14 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
   p->totalGlobalMem = size_t(16) << 30;
   return hipSuccess;
 }
+
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t attr, int) {
+  if (attr != hipDeviceAttributeMultiprocessorCount) return hipErrorInvalidValue;
+  hipDeviceProp_t p;
+  hipGetDeviceProperties(&p, 0);
+  *v = p.multiProcessorCount;
+  return hipSuccess;
+}
 hipError_t hipGetLastError() { return hipSuccess; }
 const char *hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "emulated error"; }
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) {

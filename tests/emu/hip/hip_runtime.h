@@ -36,6 +36,10 @@
 #define __HIP_MEMORY_SCOPE_AGENT 0
 #define __hip_atomic_load(p, order, scope) __atomic_load_n((p), (order))
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
+// callers pass wavefront-uniform values
+#define __builtin_amdgcn_readfirstlane(x) (x)
+// register allocation hints mean nothing here
+#define amdgpu_waves_per_eu(...) unused
 
 struct dim3 {
   uint32_t x, y, z;
@@ -108,6 +112,14 @@ hipError_t hipSetDevice(int d);
 hipError_t hipGetDevice(int *d);
 hipError_t hipGetDeviceCount(int *n);
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int d);
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t attr, int d);
+// Two workgroups per CU for every kernel.
+template <class K>
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, K, int, size_t) {
+  *n = 2;
+  return hipSuccess;
+}
 hipError_t hipGetLastError();
 const char *hipGetErrorString(hipError_t e);
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);

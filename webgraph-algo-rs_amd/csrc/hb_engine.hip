@@ -131,6 +131,7 @@ struct Instance {
   std::vector<std::unique_ptr<DevBuf<uint8_t>>> dc;
   DevBuf<uint32_t> list, bsum, list_len;
   DevBuf<Stats> stats;
+  DevBuf<uint64_t> stat_slots;  // kStatSlots lines of per-wavefront partial statistics
   DevBuf<uint32_t> work;
   Stats *h_stats = nullptr;
   uint32_t max_grid = 0;
@@ -329,6 +330,7 @@ struct Instance {
     ua.m = m;
     ua.lc_table = lc.p;
     ua.stats = stats.p;
+    ua.stat_slots = stat_slots.p;
     ua.work = work.p;
     ua.heavy_degree = heavy_count ? heavy_degree : UINT64_MAX;
     ua.heavy_list = d_heavy.p;
@@ -392,10 +394,11 @@ struct Instance {
     if (systolic) fill_zero(p.ua.next_must, bitmap_words * 4);
     if (p.list_mode)
       HB_HIP(launch_compact(p.ua.must, p.ua.curr_mod, n, bsum.p, list.p, list_len.p, stream));
-    HB_HIP(launch_reset_stats(stats.p, work.p, stream));
+    HB_HIP(launch_reset_stats(stats.p, work.p, stat_slots.p, stream));
     if (timing) HB_HIP(hipEventRecord(ev[1], stream));
     HB_HIP(launch_union(int(r), int(log2m), p.ua, p.grid, p.heavy_grid, stream));
     if (timing) HB_HIP(hipEventRecord(ev[2], stream));
+    HB_HIP(launch_reduce_stats(stats.p, stat_slots.p, stream));
     HB_HIP(hipMemcpyAsync(h_stats, stats.p, sizeof(Stats), hipMemcpyDeviceToHost, stream));
     if (timing) HB_HIP(hipEventRecord(ev[3], stream));
     HB_HIP(hipStreamSynchronize(stream));
@@ -649,6 +652,7 @@ struct Batch {
       d.u = plans[i].ua;
       d.u.stats = d_stats.p + i;
       d.u.work = work.p;
+      d.u.stat_slots = nullptr;  // the batch kernels add to their instance's Stats
       d.active = 1;
       d.list_mode = plans[i].list_mode;
       d.clear_must = inst[i]->systolic;
@@ -869,6 +873,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
       I.set_heavy_degree(Instance::kDefaultHeavy);
     }
     I.stats.alloc(1);
+    I.stat_slots.alloc(size_t(kStatSlots) * kStatSlotWords);
     I.work.alloc(kWorkCounters * kWorkStride);
     HB_HIP(hipStreamSynchronize(s));
     return inst;

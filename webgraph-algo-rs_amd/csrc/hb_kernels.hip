@@ -20,6 +20,9 @@
 // read of the cache.
 #include "hb_kernels.hpp"
 
+#include <algorithm>
+#include <atomic>
+
 #include "hb_hll.hpp"
 
 // Lane group per (r, log2m): G lanes per node and SPL slabs per lane, with
@@ -244,19 +247,25 @@ __device__ __forceinline__ void acc_add(void *base, uint32_t mode, uint32_t v, d
   }
 }
 
-// Wavefront sum of the per-lane accumulators, added to *st by lane 0.  Every
-// lane of the wavefront must call it.
-__device__ __forceinline__ void flush_stats(const Acc &a, Stats *st) {
-  uint64_t v[11] = {uint64_t(a.nf_int), uint64_t(a.nf_hi), uint64_t(a.nf_lo), a.visited, a.scanned,
-                    a.tscanned, a.checked, a.copied, a.modified, a.gathered, a.changed};
+// Wavefront sum of the per-lane accumulators, added by lane 0 to the
+// wavefront's statistics slot, or to *st when `slots` is null.  Every lane of
+// the wavefront must call it.
+__device__ __forceinline__ void flush_stats(const Acc &a, Stats *st, uint64_t *slots) {
+  uint64_t v[kStatFields] = {uint64_t(a.nf_int), uint64_t(a.nf_hi), uint64_t(a.nf_lo), a.visited,
+                             a.scanned, a.tscanned, a.checked, a.copied, a.modified, a.gathered,
+                             a.changed};
 #pragma unroll
-  for (int k = 0; k < 11; k++)
+  for (int k = 0; k < kStatFields; k++)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
   if (__lane_id() == 0) {
     unsigned long long *p = reinterpret_cast<unsigned long long *>(st);  // Stats field order
+    if (slots) {
+      const uint32_t w = (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) % kStatSlots;
+      p = reinterpret_cast<unsigned long long *>(slots + uint64_t(w) * kStatSlotWords);
+    }
 #pragma unroll
-    for (int k = 0; k < 11; k++)
+    for (int k = 0; k < kStatFields; k++)
       if (v[k]) atomicAdd(p + k, (unsigned long long)v[k]);
   }
 }
@@ -266,21 +275,40 @@ __device__ __forceinline__ void flush_stats(const Acc &a, Stats *st) {
 // blockIdx.x % 8 names the XCD a workgroup runs on).  Each XCD walks its own
 // contiguous eighth of the range, which keeps window-local successor reads in
 // that XCD's L2, and steals from the other ranges when its own is exhausted.
-// Returns UINT64_MAX when no work is left.  Called by one lane.
+// The first S chunks of each range go without atomics, one to each of the S
+// wavefronts of the XCD's full rows of workgroups (S = gridDim.x / 8 * the
+// wavefronts per workgroup; first_chunk); the counter hands out the rest,
+// from S on (grab_chunk).  Both return UINT64_MAX when no work is left and
+// are called by one lane.
+__device__ __forceinline__ uint64_t static_waves_per_xcd() {
+  return uint64_t(gridDim.x / 8) * (blockDim.x / 64);
+}
+
 __device__ __forceinline__ uint64_t grab_chunk(uint32_t *work, uint64_t chunks) {
   const uint64_t per = (chunks + 7) / 8;
   const uint32_t home = blockIdx.x % 8;
+  const uint64_t S = static_waves_per_xcd();
   for (uint32_t k = 0; k < 8; k++) {
     const uint32_t q = (home + k) % 8;
     const uint64_t base = q * per;
     if (base >= chunks) continue;
     const uint64_t lim = per < chunks - base ? per : chunks - base;
+    if (S >= lim) continue;
     uint32_t *ctr = work + q * kWorkStride;
-    if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= lim) continue;
-    const uint32_t i = atomicAdd(ctr, 1u);
+    if (S + __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= lim) continue;
+    const uint64_t i = S + atomicAdd(ctr, 1u);
     if (i < lim) return base + i;
   }
   return UINT64_MAX;
+}
+
+__device__ __forceinline__ uint64_t first_chunk(uint32_t *work, uint64_t chunks) {
+  const uint64_t per = (chunks + 7) / 8;
+  const uint64_t base = (blockIdx.x % 8) * per;
+  const uint64_t idx = uint64_t(blockIdx.x / 8) * (blockDim.x / 64) + threadIdx.x / 64;
+  if (idx < static_waves_per_xcd() && base < chunks && idx < chunks - base && idx < per)
+    return base + idx;
+  return grab_chunk(work, chunks);
 }
 
 // Orders the LDS traffic of the W lanes of a lane group (W <= 64, one
@@ -518,13 +546,25 @@ __device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int 
   }
 }
 
-// Each wavefront takes chunks of kRoundsPerChunk * (64 / G) positions and
-// gives every lane group one node per round.
+// Each wavefront takes chunks of `rounds` * (64 / G) positions and gives
+// every lane group one node per round.  A launch uses the fewest rounds, at
+// most kRoundsPerChunk, that give every wavefront of the grid at most one
+// chunk; beyond kRoundsPerChunk * (64 / G) positions per wavefront the chunks
+// are dealt dynamically.  A small node count or worklist is then spread over
+// the whole grid, every chunk static, instead of walked by a few wavefronts
+// round after round (each round is a chain of dependent loads).
 constexpr int kRoundsPerChunk = 8;
 
 template <int G>
-__host__ __device__ constexpr uint64_t chunk_positions() {
+__host__ __device__ constexpr uint64_t chunk_positions() {  // at most
   return uint64_t(kRoundsPerChunk) * (64 / G);
+}
+
+__host__ __device__ inline uint32_t chunk_rounds(uint64_t count, uint32_t nodes_per_wave,
+                                                 uint64_t waves) {
+  const uint64_t per = uint64_t(nodes_per_wave) * waves;
+  const uint64_t r = (count + per - 1) / per;
+  return r < 1 ? 1u : r > uint64_t(kRoundsPerChunk) ? uint32_t(kRoundsPerChunk) : uint32_t(r);
 }
 
 // Per-position node data of a chunk, staged in LDS by stage_chunk.
@@ -541,8 +581,8 @@ constexpr uint32_t kMetaRun = 1, kMetaCheck = 2, kMetaWasMod = 4;
 // checked nor modified, and rows left to heavy_kernel get no kMetaRun.
 template <int G>
 __device__ __forceinline__ void stage_chunk(const UnionArgs &a, uint64_t chunk, uint64_t count,
-                                            NodeMeta *meta) {
-  constexpr uint64_t CH = chunk_positions<G>();
+                                            uint32_t rounds, NodeMeta *meta) {
+  const uint64_t CH = uint64_t(rounds) * (64 / G);
   for (uint32_t p = __lane_id(); p < CH; p += 64) {
     const uint64_t i = chunk * CH + p;
     NodeMeta m{0, 0, 0, 0};
@@ -584,6 +624,7 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, const NodeMeta 
 }
 
 
+// Chunk `chunk` of `rounds` rounds (at most kRoundsPerChunk).
 // `wave_ids` and `meta` are the calling wavefront's own LDS regions, at least
 // 2 * 64 * ids_per_lane<G>() words and chunk_positions<G>() entries; its lane
 // groups take consecutive id-and-mask slots.  The regions are per wavefront, not per
@@ -591,24 +632,31 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, const NodeMeta 
 // different shapes at the same time.
 template <int R, int G, int SPL>
 __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, uint64_t count,
-                                          uint32_t *wave_ids, NodeMeta *meta, Acc &acc) {
+                                          uint32_t rounds, uint32_t *wave_ids, NodeMeta *meta,
+                                          Acc &acc) {
   constexpr int NPW = 64 / G;
   const int g = __lane_id() / G, gl = __lane_id() % G;
   uint32_t *ids = wave_ids + g * 2 * slice_width<G>();
-  stage_chunk<G>(a, chunk, count, meta);
+  stage_chunk<G>(a, chunk, count, rounds, meta);
 #pragma unroll 1
-  for (int round = 0; round < kRoundsPerChunk; round++) {
+  for (uint32_t round = 0; round < rounds; round++) {
     const NodeMeta m = meta[round * NPW + g];
     if (m.flags & kMetaRun) process_node<R, G, SPL>(a, m, gl, acc, ids);
   }
   lane_sync<64>();  // the next chunk's staging overwrites meta
 }
 
+// Shapes of 5 dwords per lane fit four wavefronts per SIMD (at most 128
+// VGPRs); the hint keeps the register allocator there.
 template <int R, int G, int SPL>
-__global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R * SPL <= 5 ? 4 : 1)))
+void union_kernel(UnionArgs a) {
   const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
-  const uint64_t chunks = (count + chunk_positions<G>() - 1) / chunk_positions<G>();
+  const uint32_t rounds = __builtin_amdgcn_readfirstlane(
+      chunk_rounds(count, 64 / G, uint64_t(gridDim.x) * (kBlock / 64)));
+  const uint64_t cp = uint64_t(rounds) * (64 / G);
+  const uint64_t chunks = (count + cp - 1) / cp;
   constexpr int kWaveIds = 2 * 64 * ids_per_lane<G>();
   constexpr int kWaveMeta = int(chunk_positions<G>());
   __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
@@ -616,14 +664,17 @@ __global__ __launch_bounds__(kBlock) void union_kernel(UnionArgs a) {
   uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
   NodeMeta *meta = s_meta + (threadIdx.x / 64) * kWaveMeta;
   Acc acc;
-  for (;;) {
-    uint64_t c = 0;
+  // the loop body stays single: a first-iteration flag inside the loop gets
+  // it peeled and costs 18 VGPRs (an occupancy step)
+  uint64_t c = 0;
+  if (__lane_id() == 0) c = first_chunk(a.work, chunks);
+  c = __shfl(c, 0);
+  while (c != UINT64_MAX) {
+    run_chunk<R, G, SPL>(a, c, count, rounds, wave_ids, meta, acc);
     if (__lane_id() == 0) c = grab_chunk(a.work, chunks);
     c = __shfl(c, 0);
-    if (c == UINT64_MAX) break;
-    run_chunk<R, G, SPL>(a, c, count, wave_ids, meta, acc);
   }
-  flush_stats(acc, a.stats);
+  flush_stats(acc, a.stats, a.stat_slots);
 }
 
 // Rows longer than heavy_degree: the whole workgroup works on one node.  Its
@@ -715,7 +766,7 @@ __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
     if (h >= a.heavy_count) break;
     heavy_node<R, G, SPL>(a, a.heavy_list[h], s_acc, s_ids, s_visit, &s_any, acc);
   }
-  flush_stats(acc, a.stats);
+  flush_stats(acc, a.stats, a.stat_slots);
 }
 
 template <int R, int G, int SPL>
@@ -762,10 +813,35 @@ __global__ __launch_bounds__(kBlock) void seed_kernel(SeedArgs a) {
   }
 }
 
-__global__ void reset_stats_kernel(Stats *st, uint32_t *work) {
+__global__ void reset_stats_kernel(Stats *st, uint32_t *work, uint64_t *slots) {
   uint32_t *p = reinterpret_cast<uint32_t *>(st);
   for (uint32_t i = threadIdx.x; i < sizeof(Stats) / 4; i += blockDim.x) p[i] = 0;
   for (uint32_t i = threadIdx.x; i < kWorkCounters * kWorkStride; i += blockDim.x) work[i] = 0;
+  if (slots)
+    for (uint32_t i = threadIdx.x; i < kStatSlots * kStatSlotWords; i += blockDim.x) slots[i] = 0;
+}
+
+// One workgroup of kStatSlots threads: thread t reads slot t; wavefront sums,
+// then the kStatSlots / 64 partial sums of each field are added in LDS.
+__global__ __launch_bounds__(kStatSlots) void reduce_stats_kernel(Stats *st,
+                                                                  const uint64_t *slots) {
+  __shared__ uint64_t part[kStatSlots / 64][kStatFields];
+  uint64_t v[kStatFields];
+#pragma unroll
+  for (int k = 0; k < kStatFields; k++) v[k] = slots[uint64_t(threadIdx.x) * kStatSlotWords + k];
+#pragma unroll
+  for (int k = 0; k < kStatFields; k++)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+  if (__lane_id() == 0)
+#pragma unroll
+    for (int k = 0; k < kStatFields; k++) part[threadIdx.x / 64][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < kStatFields) {
+    uint64_t sum = 0;
+    for (int w = 0; w < kStatSlots / 64; w++) sum += part[w][threadIdx.x];
+    reinterpret_cast<uint64_t *>(st)[threadIdx.x] = sum;  // Stats field order
+  }
 }
 
 // ---- bitmap compaction ---------------------------------------------------
@@ -898,10 +974,10 @@ __global__ void to_reference_kernel(const uint32_t *cnt, uint64_t node_begin, ui
   X(0, 5, HB_G_6, HB_SPL_6) X(1, 5, HB_G_8, HB_SPL_8) X(2, 5, HB_G_10, HB_SPL_10)             \
   X(3, 6, HB_G_6, HB_SPL_6) X(4, 6, HB_G_8, HB_SPL_8) X(5, 6, HB_G_10, HB_SPL_10)
 
-__device__ __forceinline__ uint64_t shape_chunk_positions(uint32_t shape) {
+__device__ __forceinline__ uint32_t shape_nodes_per_wave(uint32_t shape) {
   switch (shape) {
 #define X(ID, RR, GG, SS) \
-  case ID: return chunk_positions<GG>();
+  case ID: return 64 / GG;
     HB_BATCH_SHAPES(X)
 #undef X
   }
@@ -976,19 +1052,35 @@ __global__ __launch_bounds__(kBlock) void batch_compact_emit_kernel(const BatchD
   }
 }
 
+// Wavefronts of each shape's union grid.
+struct ShapeWaves {
+  uint64_t w[kBatchShapes];
+};
+
 // One thread: the positions of every active instance and, shape by shape,
-// the prefix sums of their chunks (each shape has its own chunk space).
+// the rounds per chunk (from the shape's total positions over the wavefronts
+// of its union grid) and the prefix sums of the instances' chunks (each shape
+// has its own chunk space).
 __global__ void batch_plan_kernel(BatchDesc *d, const uint32_t *shape_inst,
-                                  const uint32_t *shape_off, Stats *stats, uint64_t *total_chunks) {
+                                  const uint32_t *shape_off, Stats *stats, uint64_t *total_chunks,
+                                  ShapeWaves waves) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (int sh = 0; sh < kBatchShapes; sh++) {
+    uint64_t total = 0;
+    for (uint32_t j = shape_off[sh]; j < shape_off[sh + 1]; j++) {
+      const BatchDesc &x = d[shape_inst[j]];
+      if (x.active) total += x.list_mode ? uint64_t(*x.u.list_len) : x.n;
+    }
+    const uint32_t npw = shape_nodes_per_wave(uint32_t(sh));
+    const uint32_t rounds = chunk_rounds(total, npw, waves.w[sh]);
+    const uint64_t ch = uint64_t(rounds) * npw;
     uint64_t acc = 0;
     for (uint32_t j = shape_off[sh]; j < shape_off[sh + 1]; j++) {
       BatchDesc &x = d[shape_inst[j]];
       uint64_t count = 0;
       if (x.active) count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
       stats[shape_inst[j]].list_len = count;
-      const uint64_t ch = shape_chunk_positions(x.shape);
+      x.rounds = rounds;
       x.chunk_base = acc;
       acc += (count + ch - 1) / ch;
       x.chunk_end = acc;
@@ -1012,22 +1104,22 @@ __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d,
   const uint64_t chunks = *total_chunks;
   Acc acc;
   int64_t current = -1;
-  for (;;) {
-    uint64_t c = 0;
-    if (__lane_id() == 0) c = grab_chunk(work, chunks);
-    c = __shfl(c, 0);
-    if (c == UINT64_MAX) break;
+  uint64_t c = 0;
+  if (__lane_id() == 0) c = first_chunk(work, chunks);
+  c = __shfl(c, 0);
+  for (; c != UINT64_MAX; c = __shfl(c, 0)) {
     const uint32_t i = find_in_shape(d, inst, cnt, c, false);
     if (int64_t(i) != current) {
-      if (current >= 0) flush_stats(acc, d[current].u.stats);
+      if (current >= 0) flush_stats(acc, d[current].u.stats, nullptr);
       acc = Acc();
       current = i;
     }
     const BatchDesc &x = d[i];
     const uint64_t count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
-    run_chunk<R, G, SPL>(x.u, c - x.chunk_base, count, wave_ids, meta, acc);
+    run_chunk<R, G, SPL>(x.u, c - x.chunk_base, count, x.rounds, wave_ids, meta, acc);
+    if (__lane_id() == 0) c = grab_chunk(work, chunks);
   }
-  if (current >= 0) flush_stats(acc, d[current].u.stats);
+  if (current >= 0) flush_stats(acc, d[current].u.stats, nullptr);
 }
 
 // The heavy rows of one batched shape, one workgroup per row.
@@ -1052,14 +1144,14 @@ __global__ __launch_bounds__(kBlock) void batch_heavy_kernel(const BatchDesc *d,
     const BatchDesc &x = d[i];
     if (!x.active) continue;
     if (int64_t(i) != current) {
-      if (current >= 0) flush_stats(acc, d[current].u.stats);
+      if (current >= 0) flush_stats(acc, d[current].u.stats, nullptr);
       acc = Acc();
       current = i;
     }
     heavy_node<R, G, SPL>(x.u, x.u.heavy_list[h - x.heavy_base], s_acc, s_ids, s_visit, &s_any,
                           acc);
   }
-  if (current >= 0) flush_stats(acc, d[current].u.stats);
+  if (current >= 0) flush_stats(acc, d[current].u.stats, nullptr);
 }
 
 uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
@@ -1069,8 +1161,39 @@ uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
   return uint32_t(g);
 }
 
+// Workgroups of `kernel` resident on the current device at once (its
+// workgroups per CU, from registers and LDS, times the CUs), a multiple of 8;
+// 0 when unknown.  The kernels with static first chunks launch at most this
+// many: a workgroup beyond it would start only when others have exited, and
+// its static chunk would then run last.  Cached per kernel (`cache`) and
+// device.
+constexpr int kMaxDevices = 64;
+template <class K>
+uint32_t resident_grid(K kernel, std::atomic<uint32_t> *cache) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  uint32_t g = cache[dev].load(std::memory_order_relaxed);
+  if (g) return g;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  g = uint32_t(std::max(1, per_cu) * std::max(1, cus));
+  g = std::max<uint32_t>(8, g / 8 * 8);
+  cache[dev].store(g, std::memory_order_relaxed);
+  return g;
+}
+
 template <int R, int G, int SPL>
 hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, uint32_t heavy_grid, hipStream_t s) {
+  static std::atomic<uint32_t> cache[kMaxDevices];
+  const uint32_t cap = resident_grid(union_kernel<R, G, SPL>, cache);
+  if (cap && grid > cap) grid = cap;
   if (heavy_grid) heavy_kernel<R, G, SPL><<<heavy_grid, kBlock, 0, s>>>(a);
   union_kernel<R, G, SPL><<<grid, kBlock, 0, s>>>(a);
   return hipGetLastError();
@@ -1118,12 +1241,36 @@ int batch_shape(int r, int log2m) {
   return (r - 5) * 3 + (log2m - 6) / 2;
 }
 
+// Grid of the batched union kernel of shape `sh`: L.union_grid, capped at
+// the kernel's resident workgroups.
+static uint32_t batch_union_grid(int sh, uint32_t grid) {
+  uint32_t cap = 0;
+  switch (sh) {
+#define X(ID, RR, GG, SS)                                                       \
+  case ID: {                                                                    \
+    static std::atomic<uint32_t> cache[kMaxDevices];                            \
+    cap = resident_grid(batch_union_kernel<RR, GG, SS>, cache);                 \
+    break;                                                                      \
+  }
+    HB_BATCH_SHAPES(X)
+#undef X
+  }
+  return cap && grid > cap ? cap : grid;
+}
+
 hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
+  ShapeWaves waves{};
+  uint32_t grid[kBatchShapes];
+  for (int sh = 0; sh < kBatchShapes; sh++) {
+    grid[sh] = batch_union_grid(sh, L.union_grid);
+    waves.w[sh] = uint64_t(grid[sh]) * (kBlock / 64);
+  }
   batch_prep_kernel<<<dim3(L.prep_blocks, L.k), kBlock, 0, s>>>(L.d, L.stats, L.work);
   batch_compact_count_kernel<<<dim3(L.compact_blocks, L.k), kBlock, 0, s>>>(L.d);
   batch_compact_scan_kernel<<<L.k, 1024, 0, s>>>(L.d);
   batch_compact_emit_kernel<<<dim3(L.compact_blocks, L.k), kBlock, 0, s>>>(L.d);
-  batch_plan_kernel<<<1, 64, 0, s>>>(L.d, L.shape_inst, L.shape_off, L.stats, L.total_chunks);
+  batch_plan_kernel<<<1, 64, 0, s>>>(L.d, L.shape_inst, L.shape_off, L.stats, L.total_chunks,
+                                     waves);
   for (int sh = 0; sh < kBatchShapes; sh++) {
     if (!L.count[sh]) continue;
     const uint32_t *inst = L.shape_inst + L.first[sh];
@@ -1136,8 +1283,8 @@ hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
       batch_heavy_kernel<RR, GG, SS><<<L.heavy_grid[sh], kBlock, 0, s>>>(L.d, inst, cnt,      \
                                                                           L.total_heavy[sh],   \
                                                                           work);               \
-    batch_union_kernel<RR, GG, SS><<<L.union_grid, kBlock, 0, s>>>(L.d, inst, cnt,            \
-                                                                    L.total_chunks + sh, work); \
+    batch_union_kernel<RR, GG, SS><<<grid[sh], kBlock, 0, s>>>(L.d, inst, cnt,                \
+                                                                L.total_chunks + sh, work);     \
     break;
       HB_BATCH_SHAPES(X)
 #undef X
@@ -1162,8 +1309,13 @@ hipError_t launch_seed(const SeedArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_reset_stats(Stats *stats, uint32_t *work, hipStream_t s) {
-  reset_stats_kernel<<<1, 256, 0, s>>>(stats, work);
+hipError_t launch_reset_stats(Stats *stats, uint32_t *work, uint64_t *slots, hipStream_t s) {
+  reset_stats_kernel<<<1, 256, 0, s>>>(stats, work, slots);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_stats(Stats *stats, const uint64_t *slots, hipStream_t s) {
+  reduce_stats_kernel<<<1, kStatSlots, 0, s>>>(stats, slots);
   return hipGetLastError();
 }
 

@@ -56,6 +56,15 @@ struct Stats {
   uint64_t list_len;
 };
 
+// Statistics slots.  A kernel's wavefronts add their partial statistics (the
+// first 11 Stats fields) to slot (global wavefront index % kStatSlots), one
+// 128-byte line each, instead of all to one Stats: thousands of wavefronts
+// end at nearly the same time, and their atomics on a single line serialize.
+// launch_reduce_stats then writes the sums into the Stats.
+constexpr int kStatSlots = 256;
+constexpr int kStatSlotWords = 16;  // u64 per slot
+constexpr int kStatFields = 11;
+
 // Dynamic work counters, each on its own 256-byte line: [0, 8) one per XCD,
 // [8] heavy rows.
 constexpr int kWorkStride = 64;
@@ -97,6 +106,8 @@ struct UnionArgs {
   uint32_t m;
   const double *lc_table;  // m * ln(m / z) for z = 0..m
   Stats *stats;            // zeroed before the launch
+  uint64_t *stat_slots;    // kStatSlots * kStatSlotWords, zeroed before the launch; null:
+                           // wavefronts add to *stats directly
   uint32_t *work;          // kWorkCounters * kWorkStride words, zeroed before the launch
   uint64_t heavy_degree;         // rows longer than this go to heavy_kernel
   const uint32_t *heavy_list;    // the nodes with such rows
@@ -141,6 +152,8 @@ struct BatchDesc {
   uint32_t *bsum;  // compaction scratch
   uint64_t chunk_base, chunk_end;  // in the shape's chunk space, written by the plan step
   uint64_t heavy_base, heavy_end;  // in the shape's heavy-row space
+  uint32_t rounds;                 // rounds per chunk of the shape, written by the plan step
+  uint32_t pad;
 };
 
 // Queue counters of a batch: one region of kWorkCounters per shape.
@@ -181,9 +194,12 @@ hipError_t launch_count_all(int r, int log2m, const uint32_t *counters, uint64_t
                             double alpha_m_m, uint32_t m, const double *lc_table, double *out,
                             hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
-// Zeroes *stats and the work counters (a kernel, not a memset, so the
-// sequence can be captured in a graph).
-hipError_t launch_reset_stats(Stats *stats, uint32_t *work, hipStream_t s);
+// Zeroes *stats, the work counters and the statistics slots (nullable) (a
+// kernel, not a memset, so the sequence can be captured in a graph).
+hipError_t launch_reset_stats(Stats *stats, uint32_t *work, uint64_t *slots, hipStream_t s);
+// Writes the sums of the statistics slots into the first kStatFields fields
+// of *stats.
+hipError_t launch_reduce_stats(Stats *stats, const uint64_t *slots, hipStream_t s);
 // list <- ids of the set bits of (a | b) below n, ascending; *len <- count.
 hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *block_sums,
                           uint32_t *list, uint32_t *len, hipStream_t s);
