@@ -284,6 +284,35 @@ def bench_precision(hbm, args):
                          "f64 engine in the parity tests)", "modes": out}
 
 
+def bench_host_counters(hbm, args):
+    """Counter arrays in HBM against pinned host memory (COUNTERS_HOST, the
+    kernels read and write them over the host link): the web graph with
+    n = args.host_nodes, log2m 8, 6-bit registers, systolic frontier, sum of
+    distances and harmonic; one run to stabilisation each, after a warmup
+    run, results compared."""
+    g = hbm.Graph.web(args.host_nodes, 2.1, 3, 10000, 1000, 0.7, 2)
+    gt = g.transpose()
+    out = {"workload": f"web graph n={g.num_nodes} arcs={g.num_arcs}, log2m 8, r 6, jenkins(42)",
+           "unit": "arc-registers/s"}
+    results = []
+    for name, where in (("device", hbm.COUNTERS_DEVICE), ("host", hbm.COUNTERS_HOST)):
+        hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).register_bits(6).jenkins(42)
+              .sum_of_distances(True).sum_of_inverse_distances(True).counter_memory(where).build())
+        hb.run_until_done()
+        t0 = time.perf_counter()
+        hb.run_until_done()
+        secs = time.perf_counter() - t0
+        st = hb.iteration_stats()
+        work = sum(s["visited_arcs"] for s in st) * 256
+        out[name] = {"ms_per_run": secs * 1e3, "value": work / secs, "iterations": len(st)}
+        results.append((hb.counters(0), hb.sum_of_distances(), hb.neighborhood_function()))
+        hb.close()
+        print(f"# {name} counters: {secs * 1e3:.1f} ms per run", file=sys.stderr, flush=True)
+    out["identical_results"] = all(bool(np.array_equal(a, b)) for a, b in zip(*results))
+    out["host_over_device"] = out["host"]["ms_per_run"] / out["device"]["ms_per_run"]
+    return out
+
+
 def bench_large(hbm, args):
     """One large instance: n = 10^8 nodes, the same generator with out-degree
     Zipf(2.1) on [4, 10000] and a +-10,000 window (about 2e9 arcs), log2m 6,
@@ -357,6 +386,9 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="only the large instance (n = --large-nodes, log2m 6, r 5)")
     ap.add_argument("--large-nodes", type=int, default=100_000_000)
+    ap.add_argument("--host-counters", action="store_true",
+                    help="only the counters-in-host-memory comparison (n = --host-nodes)")
+    ap.add_argument("--host-nodes", type=int, default=1_000_000)
     ap.add_argument("--stats", action="store_true", help="print per-iteration statistics to stderr")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size "
@@ -395,6 +427,9 @@ def main():
         return
     if args.large:
         print(json.dumps({"large": bench_large(hbm, args)}), flush=True)
+        return
+    if args.host_counters:
+        print(json.dumps({"host_counters": bench_host_counters(hbm, args)}), flush=True)
         return
 
     t = time.perf_counter()

@@ -125,6 +125,15 @@ uint32_t hb_log2_num_registers(double rsd);  /* HyperLogLog::log_2_num_of_regist
 #define HB_ACC_F32_KAHAN 1  /* f32 (sum, compensation) pairs */
 #define HB_ACC_F32 2        /* plain f32 sums (precision study only) */
 
+/* Where the two counter arrays live (the role of the reference's
+ * `mmap_options: TempMmapOptions` argument, hyperball_impl.rs:89-117,
+ * mmap_helper.rs:282-293, which backs them by RAM or by temporary files). */
+#define HB_COUNTERS_DEVICE 0 /* HBM (TempMmapOptions::Default) */
+#define HB_COUNTERS_HOST 1   /* pinned host memory that the kernels read and write over the
+                                host link: instances whose counters exceed HBM; every other
+                                buffer stays in HBM.  No file-backed mode: the memory must be
+                                allocated pinned. */
+
 typedef struct {
   uint32_t log2m;        /* log2 of registers per counter (with_hyper_log_log `log2m`) */
   uint32_t reg_bits;     /* 0: HyperLogLog::register_size(num_elements); else 5 or 6 */
@@ -139,6 +148,7 @@ typedef struct {
   uint32_t acc_precision; /* HB_ACC_* */
   int device;            /* HIP device ordinal */
   int device_transpose;  /* with no transpose given: build it on the device (systolic mode) */
+  uint32_t counter_memory; /* HB_COUNTERS_* */
 } hb_config;
 
 /* Fills the defaults of HyperBallBuilder::with_hyper_log_log (:91-133):
@@ -194,6 +204,7 @@ typedef struct {
   uint32_t log2m, num_registers, register_bits, words_per_counter;
   uint64_t num_nodes, num_arcs;
   double alpha_m_m;
+  uint32_t counter_memory;  /* HB_COUNTERS_* */
 } hb_params;
 hb_status hb_get_params(hb_instance *hb, hb_params *out);
 

@@ -61,6 +61,13 @@ def test_builder_word_alignment_error(hbm):
     assert "Use u16 or smaller" in e.value.message
 
 
+def test_builder_unknown_counter_memory(hbm):
+    with pytest.raises(hbm.HyperBallError) as e:
+        hbm.HyperBallBuilder.with_hyper_log_log(_small(hbm), None, 6).counter_memory(2).build()
+    assert e.value.code == hbm.HB_E_INVALID_ARG
+    assert "unknown counter memory" in e.value.message
+
+
 def test_builder_transpose_mismatch(hbm):
     g = _small(hbm)
     t = hbm.Graph.from_arcs(6, [(1, 0)])
@@ -108,3 +115,31 @@ def test_null_arguments(hbm):
     assert lib.hb_run_until_done(None) == hbm.HB_E_INVALID_ARG
     assert b"null" in lib.hb_last_error()
     assert lib.hb_destroy(None) == hbm.HB_OK
+
+
+def test_struct_layouts_match_the_header(hbm, tmp_path):
+    """The ctypes mirrors of hb_csr, hb_graph, hb_config, hb_params and
+    hb_iteration_stats have the C layout (sizes and field offsets), as
+    compiled by gcc from include/hyperball.h."""
+    import ctypes
+    structs = {"hb_csr": hbm._Csr, "hb_graph": hbm._Graph, "hb_config": hbm._Config,
+               "hb_params": hbm._Params, "hb_iteration_stats": hbm.IterationStats}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "hyperball.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)

@@ -317,3 +317,23 @@ def test_progress_callback_and_caller_stream(hbm):
     hb.run_until_done()
     assert np.array_equal(hb.sum_of_distances(), sd_private)
     assert len(seen) == len(hb.iteration_stats())  # no calls after None
+
+
+def test_host_counter_memory_outputs(hbm):
+    """The same run with HBM and with pinned host counter arrays gives
+    identical outputs; an unknown memory kind is rejected."""
+    g, gt = _web(hbm, 20000, seed=12)
+    runs = []
+    for where in (hbm.COUNTERS_DEVICE, hbm.COUNTERS_HOST):
+        hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).jenkins(4).sum_of_distances(True)
+              .sum_of_inverse_distances(True).counter_memory(where).build())
+        assert hb.params()["counter_memory"] == where
+        hb.run_until_done()
+        runs.append((hb.counters(0), hb.sum_of_distances(), hb.harmonic_centralities(),
+                     hb.neighborhood_function()))
+        hb.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+    with pytest.raises(hbm.HyperBallError) as e:
+        hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).counter_memory(2).build()
+    assert e.value.code == hbm.HB_E_INVALID_ARG

@@ -28,11 +28,15 @@ def _tiny_graphs(hbm):
 
 
 def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_000, heavy=None,
-                  delta=None):
+                  delta=None, counter_memory=None):
     b = hbm.HyperBallBuilder.with_hyper_log_log(g, gt, log2m).register_bits(reg_bits)
     if jenkins is not None:
         b = b.jenkins(jenkins)
+    if counter_memory is not None:
+        b = b.counter_memory(counter_memory)
     hb = b.build()
+    if counter_memory is not None:
+        assert hb.params()["counter_memory"] == counter_memory
     if heavy is not None:
         hb.set_tuning(heavy)
     if delta is not None:
@@ -165,3 +169,14 @@ def test_delta_gathering_bit_exact(hbm, orc, delta, log2m, reg_bits):
         assert gathered == total
     else:
         assert gathered < total
+
+
+@pytest.mark.parametrize("log2m,reg_bits", [(6, 5), (8, 6), (10, 6), (12, 5)])
+def test_host_counter_memory_bit_exact(hbm, orc, log2m, reg_bits):
+    """Counter arrays in pinned host memory (the kernels read and write them
+    over the host link): every iteration bit-exact against the Jacobi
+    oracle, systolic and heavy rows included."""
+    g = hbm.Graph.web(6000, 2.1, 1, 1500, 100, 0.7, 3 + log2m)
+    it, stats = _run_lockstep(hbm, orc, g, g.transpose(), log2m, reg_bits, jenkins=2, heavy=128,
+                              counter_memory=hbm.COUNTERS_HOST)
+    assert it > 5 and any(s["systolic"] for s in stats)

@@ -40,26 +40,30 @@ template <class T>
 struct DevBuf {
   T *p = nullptr;
   size_t count = 0;
+  bool host = false;  // pinned host memory that kernels access over the host link
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
-  void alloc(size_t n) {
+  void alloc(size_t n, bool in_host = false) {
     release();
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(1, n) * sizeof(T));
+    const size_t bytes = std::max<size_t>(1, n) * sizeof(T);
+    hipError_t e = in_host ? hipHostMalloc(&q, bytes, hipHostMallocDefault) : hipMalloc(&q, bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();
-      fail(HB_E_OOM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
-                         hipGetErrorString(e));
+      fail(HB_E_OOM, std::string(in_host ? "hipHostMalloc" : "hipMalloc") + " of " +
+                         std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
     }
     p = static_cast<T *>(q);
     count = n;
+    host = in_host;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) (void)(host ? hipHostFree(p) : hipFree(p));
     p = nullptr;
     count = 0;
+    host = false;
   }
 };
 
@@ -731,6 +735,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     }
     if (cfg->acc_precision > HB_ACC_F32) fail(HB_E_INVALID_ARG, "unknown accumulator precision");
     if (cfg->hash > HB_HASH_JENKINS) fail(HB_E_INVALID_ARG, "unknown hash");
+    if (cfg->counter_memory > HB_COUNTERS_HOST) fail(HB_E_INVALID_ARG, "unknown counter memory");
     uint64_t num_elements = cfg->num_elements ? cfg->num_elements : n;
     if (cfg->weights && !cfg->num_elements) {
       num_elements = 0;
@@ -844,7 +849,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
 
     I.bitmap_words = std::max<uint64_t>(2, ((n + 63) / 64) * 2);
     for (int k = 0; k < 2; k++) {
-      I.cnt[k].alloc(size_t(n) * I.cstride);
+      I.cnt[k].alloc(size_t(n) * I.cstride, cfg->counter_memory == HB_COUNTERS_HOST);
       I.mod[k].alloc(I.bitmap_words);
       I.must[k].alloc(I.bitmap_words);
       I.chg[k].alloc(n);
@@ -1066,6 +1071,7 @@ hb_status hb_get_params(hb_instance *hb, hb_params *out) {
     out->num_nodes = I.n;
     out->num_arcs = I.a;
     out->alpha_m_m = I.amm;
+    out->counter_memory = I.cnt[0].host ? HB_COUNTERS_HOST : HB_COUNTERS_DEVICE;
   });
 }
 

@@ -31,6 +31,7 @@ LIB_PATH = os.path.join(_HERE, "libhyperball_amd.so")
 HB_OK, HB_E_NOT_RUN, HB_E_NOT_REQUESTED, HB_E_INVALID_ARG, HB_E_OOM, HB_E_HIP, HB_E_NO_DEVICE = range(7)
 HASH_SIPHASH13, HASH_JENKINS = 0, 1
 ACC_F64, ACC_F32_KAHAN, ACC_F32 = 0, 1, 2
+COUNTERS_DEVICE, COUNTERS_HOST = 0, 1
 
 
 class HyperBallError(RuntimeError):
@@ -57,14 +58,15 @@ class _Config(ctypes.Structure):
                 ("weights", ctypes.c_void_p), ("weight_seed", ctypes.c_uint64),
                 ("arc_granularity", ctypes.c_uint64), ("sum_of_distances", ctypes.c_int),
                 ("sum_of_inverse_distances", ctypes.c_int), ("acc_precision", ctypes.c_uint32),
-                ("device", ctypes.c_int), ("device_transpose", ctypes.c_int)]
+                ("device", ctypes.c_int), ("device_transpose", ctypes.c_int),
+                ("counter_memory", ctypes.c_uint32)]
 
 
 class _Params(ctypes.Structure):
     _fields_ = [("log2m", ctypes.c_uint32), ("num_registers", ctypes.c_uint32),
                 ("register_bits", ctypes.c_uint32), ("words_per_counter", ctypes.c_uint32),
                 ("num_nodes", ctypes.c_uint64), ("num_arcs", ctypes.c_uint64),
-                ("alpha_m_m", ctypes.c_double)]
+                ("alpha_m_m", ctypes.c_double), ("counter_memory", ctypes.c_uint32)]
 
 
 class IterationStats(ctypes.Structure):
@@ -294,6 +296,7 @@ class HyperBallBuilder:
         self._acc = ACC_F64
         self._device = 0
         self._device_transpose = False
+        self._counter_memory = COUNTERS_DEVICE
         self._weight_seed = 0
         self._num_elements = 0
         if weights is not None:
@@ -374,6 +377,14 @@ class HyperBallBuilder:
         self._device_transpose = bool(on)
         return self
 
+    def counter_memory(self, where: int) -> "HyperBallBuilder":
+        """Where the two counter arrays live (the reference's mmap_options
+        argument, hyperball_impl.rs:89-117): COUNTERS_DEVICE (HBM, default) or
+        COUNTERS_HOST (pinned host memory read and written by the kernels
+        over the host link, for counters beyond HBM)."""
+        self._counter_memory = int(where)
+        return self
+
     def accumulator_precision(self, mode: int) -> "HyperBallBuilder":
         self._acc = int(mode)
         return self
@@ -398,6 +409,7 @@ class HyperBallBuilder:
         cfg.acc_precision = self._acc
         cfg.device = self._device
         cfg.device_transpose = int(self._device_transpose)
+        cfg.counter_memory = self._counter_memory
         return cfg
 
     def build(self) -> "HyperBall":
