@@ -145,6 +145,7 @@ struct Instance {
   bool inited = false;
   uint64_t iteration = 0;
   bool systolic = false, local = false, pre_local = false;
+  bool last_full = false;  // the last iteration checked every node (not a worklist)
   double last = 0.0, rel_inc = 0.0;
   uint64_t modified = 0;
   std::vector<double> nf;
@@ -250,7 +251,7 @@ struct Instance {
     cur = mcur = mustcur = 0;
     last_changed = 0;
     iteration = 0;
-    systolic = local = pre_local = false;
+    systolic = local = pre_local = last_full = false;
     modified = 0;
     last = double(n);
     rel_inc = 0.0;
@@ -264,6 +265,7 @@ struct Instance {
   struct Plan {
     UnionArgs ua;
     bool list_mode;
+    bool mark_first;  // mark the predecessors of the modified nodes before the union
     double base_nf;
     uint32_t grid, heavy_grid;
   };
@@ -287,10 +289,20 @@ struct Instance {
       flags |= F_CHECK_ALL;
     } else if (systolic && prev_systolic) {
       p.list_mode = true;
+    } else if (systolic && last_full && !in_batch) {
+      // The first systolic iteration after one that checked every node: no
+      // predecessors were marked, so the reference checks every node
+      // again.  A node none of whose successors changed keeps its counter
+      // (the full check already merged them all), so marking the
+      // predecessors of the modified nodes first and running over the
+      // worklist gives the same counters.  Batches keep the full check.
+      p.list_mode = true;
+      p.mark_first = true;
     } else {
       flags |= F_CHECK_ALL;  // standard, or the first systolic iteration (all nodes checked)
     }
     if (p.list_mode) flags |= F_LIST;
+    last_full = !p.list_mode;
     {
       // expected fraction of successor slabs to gather: one non-zero slab per
       // counter after unweighted seeding; then the mean share of changed
@@ -396,6 +408,12 @@ struct Instance {
     if (timing) HB_HIP(hipEventRecord(ev[0], stream));
     fill_zero(p.ua.next_mod, bitmap_words * 4);
     if (systolic) fill_zero(p.ua.next_must, bitmap_words * 4);
+    if (p.mark_first) {
+      fill_zero(must[mustcur].p, bitmap_words * 4);
+      HB_HIP(launch_compact(mod[mcur].p, nullptr, n, bsum.p, list.p, list_len.p, stream));
+      HB_HIP(launch_mark_predecessors(list.p, list_len.p, toff.p, tsucc.p, must[mustcur].p,
+                                      max_grid, stream));
+    }
     if (p.list_mode)
       HB_HIP(launch_compact(p.ua.must, p.ua.curr_mod, n, bsum.p, list.p, list_len.p, stream));
     HB_HIP(launch_reset_stats(stats.p, work.p, stat_slots.p, stream));

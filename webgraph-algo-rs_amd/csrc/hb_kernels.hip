@@ -936,6 +936,25 @@ __global__ __launch_bounds__(kBlock) void compact_emit_kernel(const uint32_t *a,
   }
 }
 
+// must |= the predecessors (transpose rows) of the nodes list[0, *len): one
+// wavefront per listed node, its lanes striding over the row.
+__global__ __launch_bounds__(kBlock) void mark_predecessors_kernel(const uint32_t *list,
+                                                                   const uint32_t *len,
+                                                                   const uint64_t *toff,
+                                                                   const uint32_t *tsucc,
+                                                                   uint32_t *must) {
+  const uint64_t count = *len;
+  const uint64_t waves = uint64_t(gridDim.x) * (kBlock / 64);
+  for (uint64_t i = uint64_t(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64; i < count; i += waves) {
+    const uint32_t v = list[i];
+    const uint64_t e = toff[v + 1];
+    for (uint64_t k = toff[v] + __lane_id(); k < e; k += 64) {
+      const uint32_t p = tsucc[k];
+      atomicOr(must + (p >> 5), 1u << (p & 31));
+    }
+  }
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, uint64_t words, uint32_t value) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words;
        i += uint64_t(gridDim.x) * blockDim.x)
@@ -1335,6 +1354,13 @@ hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint
   compact_count_kernel<<<nb, kBlock, 0, s>>>(a, b, n, bsum);
   compact_scan_kernel<<<1, 1024, 0, s>>>(bsum, nb, len);
   compact_emit_kernel<<<nb, kBlock, 0, s>>>(a, b, n, bsum, list);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_predecessors(const uint32_t *list, const uint32_t *len, const uint64_t *toff,
+                                   const uint32_t *tsucc, uint32_t *must, uint32_t grid,
+                                   hipStream_t s) {
+  mark_predecessors_kernel<<<grid, kBlock, 0, s>>>(list, len, toff, tsucc, must);
   return hipGetLastError();
 }
 

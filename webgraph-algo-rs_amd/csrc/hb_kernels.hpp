@@ -204,6 +204,11 @@ hipError_t launch_reduce_stats(Stats *stats, const uint64_t *slots, hipStream_t 
 hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *block_sums,
                           uint32_t *list, uint32_t *len, hipStream_t s);
 uint64_t compact_scratch_words(uint64_t n);
+// must |= the predecessors (rows of the transpose toff/tsucc) of the nodes
+// list[0, *len); grid workgroups.
+hipError_t launch_mark_predecessors(const uint32_t *list, const uint32_t *len, const uint64_t *toff,
+                                   const uint32_t *tsucc, uint32_t *must, uint32_t grid,
+                                   hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s);
 // Transpose of the CSR graph (off[n+1], succ[a]) into caller-allocated
 // toff[n+1], tsucc[a], rows sorted (hb_transpose.hip).  Allocates and frees
