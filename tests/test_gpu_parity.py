@@ -83,6 +83,8 @@ def test_web_systolic_local_bit_exact(hbm, orc, hash_):
     it, stats = _run_lockstep(hbm, orc, g, gt, 8, jenkins=(None if hash_ == "siphash" else 42))
     modes = {(s["systolic"], s["local"], s["pre_local"]) for s in stats}
     assert any(m[0] for m in modes), "no systolic iteration was exercised"
+    first = next(i for i, s in enumerate(stats) if s["systolic"])
+    assert stats[first]["checked"] < g.num_nodes, "the first systolic iteration checked every node"
 
 
 @pytest.mark.parametrize("log2m,reg_bits", [(8, 5), (6, 6), (10, 6), (12, 6), (11, 5)])
@@ -134,6 +136,9 @@ def test_cnr_2000_par_equals_seq_every_iteration(hbm, orc, cnr):
     it, stats = _run_lockstep(hbm, orc, g, gt, 6)
     assert it > 10
     assert any(s["systolic"] for s in stats) and any(s["local"] for s in stats)
+    # the first systolic iteration ran over the marked predecessors, not every node
+    first = next(i for i, s in enumerate(stats) if s["systolic"])
+    assert not stats[first - 1]["systolic"] and stats[first]["checked"] < g.num_nodes
 
 
 @pytest.mark.parametrize("heavy", [64, 0, 2**32 - 1])
