@@ -548,6 +548,7 @@ struct Batch {
   DevBuf<BatchDesc> d_desc;
   BatchDesc *h_desc = nullptr;  // pinned
   DevBuf<Stats> d_stats;
+  DevBuf<uint64_t> slots;  // per instance: kStatSlots statistics slots
   Stats *h_stats = nullptr;     // pinned
   DevBuf<uint32_t> work;
   DevBuf<uint64_t> total_chunks;
@@ -674,7 +675,7 @@ struct Batch {
       d.u = plans[i].ua;
       d.u.stats = d_stats.p + i;
       d.u.work = work.p;
-      d.u.stat_slots = nullptr;  // the batch kernels add to their instance's Stats
+      d.u.stat_slots = slots.p + i * size_t(kStatSlots) * kStatSlotWords;
       d.active = 1;
       d.list_mode = plans[i].list_mode;
       d.clear_must = inst[i]->systolic;
@@ -1292,6 +1293,7 @@ hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transpos
     b->h_stats = static_cast<Stats *>(p);
     b->d_desc.alloc(k);
     b->d_stats.alloc(k);
+    b->slots.alloc(k * size_t(kStatSlots) * kStatSlotWords);
     b->work.alloc(kBatchWorkWords);
     b->total_chunks.alloc(kBatchShapes);
     for (size_t i = 0; i < k; i++) {

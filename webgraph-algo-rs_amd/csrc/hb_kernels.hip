@@ -823,8 +823,7 @@ __global__ void reset_stats_kernel(Stats *st, uint32_t *work, uint64_t *slots) {
 
 // One workgroup of kStatSlots threads: thread t reads slot t; wavefront sums,
 // then the kStatSlots / 64 partial sums of each field are added in LDS.
-__global__ __launch_bounds__(kStatSlots) void reduce_stats_kernel(Stats *st,
-                                                                  const uint64_t *slots) {
+__device__ __forceinline__ void reduce_slots(Stats *st, const uint64_t *slots) {
   __shared__ uint64_t part[kStatSlots / 64][kStatFields];
   uint64_t v[kStatFields];
 #pragma unroll
@@ -842,6 +841,11 @@ __global__ __launch_bounds__(kStatSlots) void reduce_stats_kernel(Stats *st,
     for (int w = 0; w < kStatSlots / 64; w++) sum += part[w][threadIdx.x];
     reinterpret_cast<uint64_t *>(st)[threadIdx.x] = sum;  // Stats field order
   }
+}
+
+__global__ __launch_bounds__(kStatSlots) void reduce_stats_kernel(Stats *st,
+                                                                  const uint64_t *slots) {
+  reduce_slots(st, slots);
 }
 
 // ---- bitmap compaction ---------------------------------------------------
@@ -1027,6 +1031,9 @@ __global__ __launch_bounds__(kBlock) void batch_prep_kernel(const BatchDesc *d, 
     for (uint32_t i = threadIdx.x; i < sizeof(Stats) / 4; i += blockDim.x) p[i] = 0;
     if (blockIdx.y == 0)
       for (uint32_t i = threadIdx.x; i < kBatchWorkWords; i += blockDim.x) work[i] = 0;
+    if (x.active && x.u.stat_slots)
+      for (uint32_t i = threadIdx.x; i < kStatSlots * kStatSlotWords; i += blockDim.x)
+        x.u.stat_slots[i] = 0;
   }
   if (!x.active) return;
   for (uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < x.words;
@@ -1071,6 +1078,14 @@ __global__ __launch_bounds__(kBlock) void batch_compact_emit_kernel(const BatchD
   }
 }
 
+// One workgroup per instance: the statistics slots of an active instance
+// summed into its Stats.
+__global__ __launch_bounds__(kStatSlots) void batch_reduce_stats_kernel(const BatchDesc *d) {
+  const BatchDesc &x = d[blockIdx.x];
+  if (!x.active || !x.u.stat_slots) return;
+  reduce_slots(x.u.stats, x.u.stat_slots);
+}
+
 // Wavefronts of each shape's union grid.
 struct ShapeWaves {
   uint64_t w[kBatchShapes];
@@ -1110,8 +1125,11 @@ __global__ void batch_plan_kernel(BatchDesc *d, const uint32_t *shape_inst,
 
 // The union step of one batched shape: per-XCD queues over the shape's chunk
 // space; a wavefront finds the instance of its chunk and runs it.
+// The hint keeps every shape at three wavefronts per SIMD (at most 168 VGPRs,
+// no spills); left alone, log2m 10 with 6-bit registers takes 175.
 template <int R, int G, int SPL>
-__global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d, const uint32_t *inst,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void batch_union_kernel(const BatchDesc *d, const uint32_t *inst,
                                                              int cnt, const uint64_t *total_chunks,
                                                              uint32_t *work) {
   constexpr int kWaveIds = 2 * 64 * ids_per_lane<G>();
@@ -1129,7 +1147,7 @@ __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d,
   for (; c != UINT64_MAX; c = __shfl(c, 0)) {
     const uint32_t i = find_in_shape(d, inst, cnt, c, false);
     if (int64_t(i) != current) {
-      if (current >= 0) flush_stats(acc, d[current].u.stats, nullptr);
+      if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
       acc = Acc();
       current = i;
     }
@@ -1138,7 +1156,7 @@ __global__ __launch_bounds__(kBlock) void batch_union_kernel(const BatchDesc *d,
     run_chunk<R, G, SPL>(x.u, c - x.chunk_base, count, x.rounds, wave_ids, meta, acc);
     if (__lane_id() == 0) c = grab_chunk(work, chunks);
   }
-  if (current >= 0) flush_stats(acc, d[current].u.stats, nullptr);
+  if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
 }
 
 // The heavy rows of one batched shape, one workgroup per row.
@@ -1163,14 +1181,14 @@ __global__ __launch_bounds__(kBlock) void batch_heavy_kernel(const BatchDesc *d,
     const BatchDesc &x = d[i];
     if (!x.active) continue;
     if (int64_t(i) != current) {
-      if (current >= 0) flush_stats(acc, d[current].u.stats, nullptr);
+      if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
       acc = Acc();
       current = i;
     }
     heavy_node<R, G, SPL>(x.u, x.u.heavy_list[h - x.heavy_base], s_acc, s_ids, s_visit, &s_any,
                           acc);
   }
-  if (current >= 0) flush_stats(acc, d[current].u.stats, nullptr);
+  if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
 }
 
 uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
@@ -1309,6 +1327,7 @@ hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
 #undef X
     }
   }
+  batch_reduce_stats_kernel<<<L.k, kStatSlots, 0, s>>>(L.d);
   return hipGetLastError();
 }
 
