@@ -315,6 +315,7 @@ struct Instance {
               (delta_mode == 0 && log2m >= kDeltaMinLog2m && est < kDeltaMaxFraction);
       if (delta) flags |= F_DELTA;
     }
+    if (iteration == 0 && !weighted) flags |= F_SEEDED;
 
     UnionArgs &ua = p.ua;
     ua.off = off.p;
@@ -344,6 +345,10 @@ struct Instance {
     ua.dist = double(iteration + 1);
     ua.alpha_m_m = amm;
     ua.m = m;
+    ua.log2m = log2m;
+    ua.hash = hash;
+    ua.seed = seed;
+    ua.sentinel = sentinel;
     ua.lc_table = lc.p;
     ua.stats = stats.p;
     ua.stat_slots = stat_slots.p;

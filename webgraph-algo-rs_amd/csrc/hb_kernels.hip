@@ -335,6 +335,16 @@ __host__ __device__ constexpr int slice_width() {
   return G * ids_per_lane<G>();
 }
 
+// The register a node's counter gets from seeding item `item` (no weights:
+// the node id): index j and value val (hyper_log_log_logic.rs:151-167).
+__device__ __forceinline__ void seed_register(uint32_t hash, uint64_t seed, uint32_t log2m,
+                                              uint64_t sentinel, uint64_t item, uint32_t &j,
+                                              uint32_t &val) {
+  const uint64_t x = hash == 1 ? jenkins(item, seed) : siphash13(item);
+  j = uint32_t(x & ((uint64_t(1) << log2m) - 1));
+  val = uint32_t(__builtin_ctzll((x >> log2m) | sentinel)) + 1;
+}
+
 // Merges into `cur` the counters of the modified successors (other than v)
 // found in the slices of v's row that start at first, first + stride, ...
 // (hyperball_impl.rs:1037-1049).  `prev` / `cur` are loaded from the node's
@@ -352,12 +362,21 @@ __host__ __device__ constexpr int slice_width() {
 // modified in iteration t - 2, so it still dominates their unchanged
 // counters.  So only registers that changed in iteration t - 1 can raise it.
 // In iteration 0 the mask holds the slabs with a non-zero register.
-template <int R, int G, int SPL, bool DELTA>
+//
+// Seeded counters (MODE 2, iteration 0 without weights): every successor's
+// counter holds the single register of its own hash, so each lane hashes the
+// ids it read, packs (register, value) into the slot instead of the id, and
+// the lane owning that register's slab takes the maximum.  No successor
+// counter or change mask is read.
+//
+// MODE: 0 whole counters, 1 delta gathering, 2 seeded counters.
+template <int R, int G, int SPL, int MODE>
 __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int gl,
                                                  uint64_t first, uint64_t end, uint64_t stride,
                                                  uint32_t *prev, uint32_t *cur, bool &loaded,
                                                  uint32_t &nvisit, uint32_t &acc_gathered,
                                                  uint32_t *ids) {
+  constexpr bool DELTA = MODE == 1, SEEDED = MODE == 2;
   constexpr int D = R * SPL;         // dwords per lane
   constexpr int CD = G * D;          // dwords per counter
   constexpr int K = ids_per_lane<G>();
@@ -392,6 +411,14 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
         f[j] = (s[j] != v) && test_bit(a.curr_mod, s[j]);
         mk[j] = ~0u;
       }
+      if (SEEDED) {
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+          uint32_t reg = 0, val = 0;
+          if (f[j]) seed_register(a.hash, a.seed, a.log2m, a.sentinel, s[j], reg, val);
+          mk[j] = reg | (val << 16);
+        }
+      }
     } else if (speculate) {
 #pragma unroll
       for (int j = 0; j < K; j++) mk[j] = s[j] != v ? a.chg_cur[s[j]] : 0u;
@@ -409,7 +436,7 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
       const uint64_t mask = group_ballot<G>(f[j]);
       if (f[j]) {
         const uint32_t pos = cnt + __popcll(mask & below);
-        ids[pos] = s[j];
+        ids[pos] = SEEDED ? mk[j] : s[j];
         if (DELTA) masks[pos] = mk[j];
       }
       cnt += __popcll(mask);
@@ -423,6 +450,26 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
       loaded = true;
     }
     nvisit += cnt;
+    if (SEEDED) {
+      for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t w = ids[k];
+        const uint32_t reg = w & 0xffffu, val = w >> 16, slab = reg >> 5, bit = reg & 31;
+#pragma unroll
+        for (int q = 0; q < SPL; q++) {
+          if (slab != uint32_t(gl) * SPL + q) continue;
+          uint32_t cv = 0;
+#pragma unroll
+          for (int b = 0; b < R; b++) cv |= ((cur[q * R + b] >> bit) & 1u) << b;
+          if (val > cv) {
+#pragma unroll
+            for (int b = 0; b < R; b++)
+              cur[q * R + b] = (cur[q * R + b] & ~(1u << bit)) | (((val >> b) & 1u) << bit);
+          }
+        }
+      }
+      lane_sync<G>();
+      continue;
+    }
     for (uint32_t k = 0; k < cnt; k += U) {
       uint32_t y[U][D];
 #pragma unroll
@@ -463,12 +510,15 @@ __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v,
                                                  uint32_t *prev, uint32_t *cur, bool &loaded,
                                                  uint32_t &nvisit, uint32_t &acc_gathered,
                                                  uint32_t *ids) {
-  if (a.flags & F_DELTA)
-    merge_slices<R, G, SPL, true>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                                  acc_gathered, ids);
+  if (a.flags & F_SEEDED)
+    merge_slices<R, G, SPL, 2>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                               acc_gathered, ids);
+  else if (a.flags & F_DELTA)
+    merge_slices<R, G, SPL, 1>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                               acc_gathered, ids);
   else
-    merge_slices<R, G, SPL, false>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                                   acc_gathered, ids);
+    merge_slices<R, G, SPL, 0>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                               acc_gathered, ids);
 }
 
 // Everything after the successor loop (hyperball_impl.rs:1051-1144): changed
@@ -702,12 +752,15 @@ __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint3
     const uint64_t first = beg + uint64_t(g) * slice_width<G>();
     const uint64_t stride = uint64_t(NG) * slice_width<G>();
     uint32_t *slot = s_ids + g * 2 * slice_width<G>();
-    if (DELTA_OK && (a.flags & F_DELTA))
-      merge_slices<R, G, SPL, true>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                                    gathered, slot);
+    if (DELTA_OK && (a.flags & F_SEEDED))
+      merge_slices<R, G, SPL, 2>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                 gathered, slot);
+    else if (DELTA_OK && (a.flags & F_DELTA))
+      merge_slices<R, G, SPL, 1>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                 gathered, slot);
     else
-      merge_slices<R, G, SPL, false>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                                     gathered, slot);
+      merge_slices<R, G, SPL, 0>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                 gathered, slot);
   }
   acc.gathered += gathered;
   if (threadIdx.x == 0) *s_any = 0;
@@ -797,9 +850,8 @@ __global__ __launch_bounds__(kBlock) void seed_kernel(SeedArgs a) {
     uint32_t chg = 0;
     for (uint64_t k = 0; k < items; k++) {
       const uint64_t item = a.weights ? weight_item(a.weight_seed, first + k) : v;
-      const uint64_t x = a.hash == 1 ? jenkins(item, a.seed) : siphash13(item);
-      const uint32_t j = uint32_t(x & ((uint64_t(1) << a.log2m) - 1));
-      const uint32_t val = uint32_t(__builtin_ctzll((x >> a.log2m) | a.sentinel)) + 1;
+      uint32_t j = 0, val = 0;
+      seed_register(a.hash, a.seed, a.log2m, a.sentinel, item, j, val);
       chg |= 1u << ((j >> 5) >> a.chg_shift);
       uint32_t *slab = c + (j >> 5) * a.r;
       const uint32_t q = j & 31;

@@ -38,6 +38,8 @@ enum UnionFlags : uint32_t {
   F_CENTRALITY = 4u,  // update the per-node accumulators
   F_LIST = 8u,        // nodes come from a compacted list whose length lives on the device
   F_DELTA = 16u,      // gather only the changed slabs of successors (change masks)
+  F_SEEDED = 32u,     // counters are as seeded without weights: each holds the one register
+                      // of its node's hash, computed instead of read
 };
 
 enum AccMode : uint32_t { ACC_F64 = 0, ACC_F32_KAHAN = 1, ACC_F32 = 2 };
@@ -104,6 +106,8 @@ struct UnionArgs {
   double dist;  // iteration + 1
   double alpha_m_m;
   uint32_t m;
+  uint32_t log2m, hash;    // F_SEEDED: the seeding hash (HB_HASH_*), its seed and the
+  uint64_t seed, sentinel; // register-value sentinel, as in SeedArgs
   const double *lc_table;  // m * ln(m / z) for z = 0..m
   Stats *stats;            // zeroed before the launch
   uint64_t *stat_slots;    // kStatSlots * kStatSlotWords, zeroed before the launch; null:
