@@ -223,7 +223,10 @@ struct Instance {
 
   void init() {
     use_device();
-    for (auto &c : cnt) fill_zero(c.p, size_t(n) * cstride * 4);
+    // cnt[1] starts zeroed; cnt[0] is zeroed for seed_kernel (weights) or
+    // written whole by the fused seeding and count (no weights)
+    fill_zero(cnt[1].p, size_t(n) * cstride * 4);
+    if (weighted) fill_zero(cnt[0].p, size_t(n) * cstride * 4);
     SeedArgs s{};
     s.counters = cnt[0].p;
     s.n = n;
@@ -238,8 +241,12 @@ struct Instance {
     s.weight_seed = weight_seed;
     s.chg = chg[0].p;
     s.chg_shift = chg_shift;
-    HB_HIP(launch_seed(s, stream));
-    HB_HIP(launch_count_all(int(r), int(log2m), cnt[0].p, n, amm, m, lc.p, cache.p, stream));
+    if (weighted) {
+      HB_HIP(launch_seed(s, stream));
+      HB_HIP(launch_count_all(int(r), int(log2m), cnt[0].p, n, amm, m, lc.p, cache.p, stream));
+    } else {
+      HB_HIP(launch_seed_count(s, amm, m, lc.p, cache.p, stream));
+    }
     if (do_sd) fill_zero(sd.p, n * acc_bytes());
     if (do_hc) fill_zero(hc.p, n * acc_bytes());
     for (auto &d : dc) fill_zero(d->p, n * acc_bytes());

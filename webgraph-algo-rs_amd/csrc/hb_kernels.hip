@@ -822,6 +822,39 @@ __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
   flush_stats(acc, a.stats, a.stat_slots);
 }
 
+// Seeding without weights, fused with the count: G lanes per node write the
+// whole counter (zero but for the register of the node's hash, as
+// seed_kernel leaves it in a zeroed array), take count() of it with the same
+// group_count as count_all_kernel, and write the change mask.  Padding dwords
+// beyond the counter are not written; nothing reads them.
+template <int R, int G, int SPL>
+__global__ __launch_bounds__(kBlock) void seed_count_kernel(SeedArgs a, double alpha_m_m,
+                                                           uint32_t m, const double *lc,
+                                                           double *out) {
+  constexpr int D = R * SPL, NPB = kBlock / G;
+  const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  for (uint64_t batch = blockIdx.x; batch * NPB < a.n; batch += gridDim.x) {
+    const uint64_t i = batch * NPB + g;
+    if (i < a.n) {
+      uint32_t reg = 0, val = 0;
+      seed_register(a.hash, a.seed, a.log2m, a.sentinel, i, reg, val);
+      uint32_t x[D];
+#pragma unroll
+      for (int q = 0; q < SPL; q++) {
+        const bool mine = (reg >> 5) == uint32_t(gl) * SPL + q;
+#pragma unroll
+        for (int b = 0; b < R; b++) x[q * R + b] = mine && ((val >> b) & 1u) ? 1u << (reg & 31) : 0u;
+      }
+      store_part<D>(a.counters + i * a.cdw + gl * D, x);
+      const double c = group_count<R, G, SPL>(x, alpha_m_m, m, lc);
+      if (gl == 0) {
+        out[i] = c;
+        a.chg[i] = 1u << ((reg >> 5) >> a.chg_shift);
+      }
+    }
+  }
+}
+
 template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) void count_all_kernel(const uint32_t *cnt, uint64_t n,
                                                           double alpha_m_m, uint32_t m,
@@ -1289,6 +1322,13 @@ hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, uint32_t heavy_grid
 }
 
 template <int R, int G, int SPL>
+hipError_t launch_seed_count_t(const SeedArgs &a, double amm, uint32_t m, const double *lc,
+                               double *out, hipStream_t s) {
+  seed_count_kernel<R, G, SPL><<<grid_for(a.n, kBlock / G, 65536), kBlock, 0, s>>>(a, amm, m, lc, out);
+  return hipGetLastError();
+}
+
+template <int R, int G, int SPL>
 hipError_t launch_count_t(const uint32_t *c, uint64_t n, double amm, uint32_t m, const double *lc,
                           double *out, hipStream_t s) {
   count_all_kernel<R, G, SPL><<<grid_for(n, kBlock / G, 65536), kBlock, 0, s>>>(c, n, amm, m, lc, out);
@@ -1388,6 +1428,17 @@ hipError_t launch_count_all(int r, int log2m, const uint32_t *c, uint64_t n, dou
   if (n == 0) return hipSuccess;
 #define X(RR, LM, GG, SS) \
   if (r == RR && log2m == LM) return launch_count_t<RR, GG, SS>(c, n, amm, m, lc, out, s);
+  HB_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_seed_count(const SeedArgs &a, double amm, uint32_t m, const double *lc,
+                             double *out, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.weights) return hipErrorInvalidValue;
+#define X(RR, LM, GG, SS) \
+  if (int(a.r) == RR && int(a.log2m) == LM) return launch_seed_count_t<RR, GG, SS>(a, amm, m, lc, out, s);
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
