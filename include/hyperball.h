@@ -210,7 +210,8 @@ hb_status hb_get_params(hb_instance *hb, hb_params *out);
 
 /* Counter arrays in the reference layout (slice_counter_array.rs:166-169,
  * hyper_log_log_logic.rs:72-115): counter v is words [v*W, (v+1)*W), register
- * j at bit j*r, LSB first.  which = 0: current state, 1: the other buffer. */
+ * j at bit j*r, LSB first.  which = 0: current state, 1: the other buffer
+ * (all zero before the first iteration). */
 hb_status hb_get_counters(hb_instance *hb, int which, uint64_t *out, size_t words);
 /* Modified-counter bits of the last iteration (AtomicBitVec layout: bit v of
  * word v/64). */

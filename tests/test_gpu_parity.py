@@ -46,6 +46,7 @@ def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_
     ref = orc.Jacobi(hll, g.offsets, g.successors)
     hb.init()
     assert np.array_equal(hb.counters(0), ref.curr), "seeded registers differ"
+    assert not hb.counters(1).any(), "the other buffer is not zero before the first iteration"
     it = 0
     while it < max_iter:
         hb.iterate()

@@ -223,9 +223,10 @@ struct Instance {
 
   void init() {
     use_device();
-    // cnt[1] starts zeroed; cnt[0] is zeroed for seed_kernel (weights) or
-    // written whole by the fused seeding and count (no weights)
-    fill_zero(cnt[1].p, size_t(n) * cstride * 4);
+    // cnt[0] is zeroed for seed_kernel (weights) or written whole by the
+    // fused seeding and count (no weights).  cnt[1] is not initialised:
+    // iteration 0 writes every node's counter there (every node starts
+    // modified), and hb_get_counters reports it as zero until then.
     if (weighted) fill_zero(cnt[0].p, size_t(n) * cstride * 4);
     SeedArgs s{};
     s.counters = cnt[0].p;
@@ -1112,6 +1113,10 @@ hb_status hb_get_counters(hb_instance *hb, int which, uint64_t *out, size_t word
     if (!out) fail(HB_E_INVALID_ARG, "null output");
     if (!I.inited) fail(HB_E_NOT_RUN, "counters are not initialised; call hb_init or hb_run");
     if (words < I.n * I.W) fail(HB_E_INVALID_ARG, "output buffer too small");
+    if (which && I.iteration == 0) {  // the other buffer before the first iteration
+      std::memset(out, 0, I.n * I.W * sizeof(uint64_t));
+      return;
+    }
     I.use_device();
     const uint32_t *src = I.cnt[which ? 1 - I.cur : I.cur].p;
     const uint64_t chunk = std::max<uint64_t>(1, (uint64_t(64) << 20) / (I.W * 8));
