@@ -246,7 +246,7 @@ struct Instance {
       HB_HIP(launch_seed(s, stream));
       HB_HIP(launch_count_all(int(r), int(log2m), cnt[0].p, n, amm, m, lc.p, cache.p, stream));
     } else {
-      HB_HIP(launch_seed_count(s, amm, m, lc.p, cache.p, stream));
+      HB_HIP(launch_seed_count(s, m, lc.p, cache.p, stream));
     }
     if (do_sd) fill_zero(sd.p, n * acc_bytes());
     if (do_hc) fill_zero(hc.p, n * acc_bytes());

@@ -824,13 +824,15 @@ __global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
 
 // Seeding without weights, fused with the count: G lanes per node write the
 // whole counter (zero but for the register of the node's hash, as
-// seed_kernel leaves it in a zeroed array), take count() of it with the same
-// group_count as count_all_kernel, and write the change mask.  Padding dwords
-// beyond the counter are not written; nothing reads them.
+// seed_kernel leaves it in a zeroed array) and the change mask, and store
+// count() of it.  With one non-zero register a counter has m - 1 zeros and a
+// harmonic sum above m - 1, so the raw estimate alpha_m_m / sum stays below
+// 0.7214 (m + 2) < 2.5 m: count() is the linear-counting value
+// lc[m - 1] for every node, as group_count returns it.  Padding dwords beyond
+// the counter are not written; nothing reads them.
 template <int R, int G, int SPL>
-__global__ __launch_bounds__(kBlock) void seed_count_kernel(SeedArgs a, double alpha_m_m,
-                                                           uint32_t m, const double *lc,
-                                                           double *out) {
+__global__ __launch_bounds__(kBlock) void seed_count_kernel(SeedArgs a, uint32_t m,
+                                                           const double *lc, double *out) {
   constexpr int D = R * SPL, NPB = kBlock / G;
   const int g = threadIdx.x / G, gl = threadIdx.x % G;
   for (uint64_t batch = blockIdx.x; batch * NPB < a.n; batch += gridDim.x) {
@@ -846,9 +848,8 @@ __global__ __launch_bounds__(kBlock) void seed_count_kernel(SeedArgs a, double a
         for (int b = 0; b < R; b++) x[q * R + b] = mine && ((val >> b) & 1u) ? 1u << (reg & 31) : 0u;
       }
       store_part<D>(a.counters + i * a.cdw + gl * D, x);
-      const double c = group_count<R, G, SPL>(x, alpha_m_m, m, lc);
       if (gl == 0) {
-        out[i] = c;
+        out[i] = lc[m - 1];
         a.chg[i] = 1u << ((reg >> 5) >> a.chg_shift);
       }
     }
@@ -1322,9 +1323,9 @@ hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, uint32_t heavy_grid
 }
 
 template <int R, int G, int SPL>
-hipError_t launch_seed_count_t(const SeedArgs &a, double amm, uint32_t m, const double *lc,
-                               double *out, hipStream_t s) {
-  seed_count_kernel<R, G, SPL><<<grid_for(a.n, kBlock / G, 65536), kBlock, 0, s>>>(a, amm, m, lc, out);
+hipError_t launch_seed_count_t(const SeedArgs &a, uint32_t m, const double *lc, double *out,
+                               hipStream_t s) {
+  seed_count_kernel<R, G, SPL><<<grid_for(a.n, kBlock / G, 65536), kBlock, 0, s>>>(a, m, lc, out);
   return hipGetLastError();
 }
 
@@ -1433,12 +1434,12 @@ hipError_t launch_count_all(int r, int log2m, const uint32_t *c, uint64_t n, dou
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_seed_count(const SeedArgs &a, double amm, uint32_t m, const double *lc,
-                             double *out, hipStream_t s) {
+hipError_t launch_seed_count(const SeedArgs &a, uint32_t m, const double *lc, double *out,
+                             hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   if (a.weights) return hipErrorInvalidValue;
 #define X(RR, LM, GG, SS) \
-  if (int(a.r) == RR && int(a.log2m) == LM) return launch_seed_count_t<RR, GG, SS>(a, amm, m, lc, out, s);
+  if (int(a.r) == RR && int(a.log2m) == LM) return launch_seed_count_t<RR, GG, SS>(a, m, lc, out, s);
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;

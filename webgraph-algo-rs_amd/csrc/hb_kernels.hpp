@@ -199,9 +199,11 @@ hipError_t launch_count_all(int r, int log2m, const uint32_t *counters, uint64_t
                             hipStream_t s);
 hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
 // Seeding without weights (a.weights null) into a counter array that need not
-// be zeroed, fused with count() of every counter into out and the change masks.
-hipError_t launch_seed_count(const SeedArgs &a, double alpha_m_m, uint32_t m, const double *lc,
-                             double *out, hipStream_t s);
+// be zeroed, fused with count() of every counter into out (lc[m - 1], the
+// linear-counting value of a counter with one non-zero register) and the
+// change masks.
+hipError_t launch_seed_count(const SeedArgs &a, uint32_t m, const double *lc, double *out,
+                             hipStream_t s);
 // Zeroes *stats, the work counters and the statistics slots (nullable) (a
 // kernel, not a memset, so the sequence can be captured in a graph).
 hipError_t launch_reset_stats(Stats *stats, uint32_t *work, uint64_t *slots, hipStream_t s);
