@@ -419,8 +419,8 @@ struct Instance {
     use_device();
     Plan p = plan_iteration();
     if (timing) HB_HIP(hipEventRecord(ev[0], stream));
-    fill_zero(p.ua.next_mod, bitmap_words * 4);
-    if (systolic) fill_zero(p.ua.next_must, bitmap_words * 4);
+    HB_HIP(launch_prepare_iteration(p.ua.next_mod, systolic ? p.ua.next_must : nullptr,
+                                    bitmap_words, stats.p, work.p, stat_slots.p, stream));
     if (p.mark_first) {
       fill_zero(must[mustcur].p, bitmap_words * 4);
       HB_HIP(launch_compact(mod[mcur].p, nullptr, n, bsum.p, list.p, list_len.p, stream));
@@ -429,12 +429,10 @@ struct Instance {
     }
     if (p.list_mode)
       HB_HIP(launch_compact(p.ua.must, p.ua.curr_mod, n, bsum.p, list.p, list_len.p, stream));
-    HB_HIP(launch_reset_stats(stats.p, work.p, stat_slots.p, stream));
     if (timing) HB_HIP(hipEventRecord(ev[1], stream));
     HB_HIP(launch_union(int(r), int(log2m), p.ua, p.grid, p.heavy_grid, stream));
     if (timing) HB_HIP(hipEventRecord(ev[2], stream));
-    HB_HIP(launch_reduce_stats(stats.p, stat_slots.p, stream));
-    HB_HIP(hipMemcpyAsync(h_stats, stats.p, sizeof(Stats), hipMemcpyDeviceToHost, stream));
+    HB_HIP(launch_reduce_stats(stats.p, stat_slots.p, h_stats, stream));  // h_stats: pinned
     if (timing) HB_HIP(hipEventRecord(ev[3], stream));
     HB_HIP(hipStreamSynchronize(stream));
     float ums = 0, ims = 0;

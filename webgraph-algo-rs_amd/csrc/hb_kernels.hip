@@ -899,7 +899,20 @@ __global__ __launch_bounds__(kBlock) void seed_kernel(SeedArgs a) {
   }
 }
 
-__global__ void reset_stats_kernel(Stats *st, uint32_t *work, uint64_t *slots) {
+// Start of an iteration in one launch: clears next_mod and (when non-null)
+// next_must, `words` words each; workgroup 0 also zeroes the statistics, the
+// work counters and the statistics slots (nullable).
+__global__ __launch_bounds__(kBlock) void prepare_iteration_kernel(uint32_t *next_mod,
+                                                                   uint32_t *next_must,
+                                                                   uint64_t words, Stats *st,
+                                                                   uint32_t *work,
+                                                                   uint64_t *slots) {
+  for (uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x; w < words;
+       w += uint64_t(gridDim.x) * kBlock) {
+    next_mod[w] = 0;
+    if (next_must) next_must[w] = 0;
+  }
+  if (blockIdx.x != 0) return;
   uint32_t *p = reinterpret_cast<uint32_t *>(st);
   for (uint32_t i = threadIdx.x; i < sizeof(Stats) / 4; i += blockDim.x) p[i] = 0;
   for (uint32_t i = threadIdx.x; i < kWorkCounters * kWorkStride; i += blockDim.x) work[i] = 0;
@@ -929,9 +942,14 @@ __device__ __forceinline__ void reduce_slots(Stats *st, const uint64_t *slots) {
   }
 }
 
-__global__ __launch_bounds__(kStatSlots) void reduce_stats_kernel(Stats *st,
-                                                                  const uint64_t *slots) {
-  reduce_slots(st, slots);
+// out <- the slot sums, and st's list_len (written by the union kernel).  out
+// may be pinned host memory: the caller then reads it after synchronising,
+// with no copy.
+__global__ __launch_bounds__(kStatSlots) void reduce_stats_kernel(const Stats *st,
+                                                                  const uint64_t *slots,
+                                                                  Stats *out) {
+  reduce_slots(out, slots);
+  if (threadIdx.x == kStatFields) out->list_len = st->list_len;
 }
 
 // ---- bitmap compaction ---------------------------------------------------
@@ -1451,13 +1469,17 @@ hipError_t launch_seed(const SeedArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_reset_stats(Stats *stats, uint32_t *work, uint64_t *slots, hipStream_t s) {
-  reset_stats_kernel<<<1, 256, 0, s>>>(stats, work, slots);
+hipError_t launch_prepare_iteration(uint32_t *next_mod, uint32_t *next_must, uint64_t words,
+                                    Stats *stats, uint32_t *work, uint64_t *slots, hipStream_t s) {
+  prepare_iteration_kernel<<<grid_for(words, kBlock, 16384), kBlock, 0, s>>>(next_mod, next_must,
+                                                                            words, stats, work,
+                                                                            slots);
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_stats(Stats *stats, const uint64_t *slots, hipStream_t s) {
-  reduce_stats_kernel<<<1, kStatSlots, 0, s>>>(stats, slots);
+hipError_t launch_reduce_stats(const Stats *stats, const uint64_t *slots, Stats *out,
+                               hipStream_t s) {
+  reduce_stats_kernel<<<1, kStatSlots, 0, s>>>(stats, slots, out);
   return hipGetLastError();
 }
 

@@ -204,12 +204,15 @@ hipError_t launch_seed(const SeedArgs &a, hipStream_t s);
 // change masks.
 hipError_t launch_seed_count(const SeedArgs &a, uint32_t m, const double *lc, double *out,
                              hipStream_t s);
-// Zeroes *stats, the work counters and the statistics slots (nullable) (a
-// kernel, not a memset, so the sequence can be captured in a graph).
-hipError_t launch_reset_stats(Stats *stats, uint32_t *work, uint64_t *slots, hipStream_t s);
-// Writes the sums of the statistics slots into the first kStatFields fields
-// of *stats.
-hipError_t launch_reduce_stats(Stats *stats, const uint64_t *slots, hipStream_t s);
+// Start of an iteration, one kernel (not memsets, so the sequence can be
+// captured in a graph): zeroes next_mod and next_must (nullable), `words`
+// words each, *stats, the work counters and the statistics slots (nullable).
+hipError_t launch_prepare_iteration(uint32_t *next_mod, uint32_t *next_must, uint64_t words,
+                                    Stats *stats, uint32_t *work, uint64_t *slots, hipStream_t s);
+// *out <- the sums of the statistics slots (the first kStatFields fields) and
+// stats->list_len.  out may be pinned host memory.
+hipError_t launch_reduce_stats(const Stats *stats, const uint64_t *slots, Stats *out,
+                               hipStream_t s);
 // list <- ids of the set bits of (a | b) below n, ascending; *len <- count.
 hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *block_sums,
                           uint32_t *list, uint32_t *len, hipStream_t s);
