@@ -297,13 +297,13 @@ struct Instance {
       flags |= F_CHECK_ALL;
     } else if (systolic && prev_systolic) {
       p.list_mode = true;
-    } else if (systolic && last_full && !in_batch) {
+    } else if (systolic && last_full) {
       // The first systolic iteration after one that checked every node: no
       // predecessors were marked, so the reference checks every node
       // again.  A node none of whose successors changed keeps its counter
       // (the full check already merged them all), so marking the
       // predecessors of the modified nodes first and running over the
-      // worklist gives the same counters.  Batches keep the full check.
+      // worklist gives the same counters.
       p.list_mode = true;
       p.mark_first = true;
     } else {
@@ -689,6 +689,7 @@ struct Batch {
       d.u.stat_slots = slots.p + i * size_t(kStatSlots) * kStatSlotWords;
       d.active = 1;
       d.list_mode = plans[i].list_mode;
+      d.mark_first = plans[i].mark_first;
       d.clear_must = inst[i]->systolic;
     }
     if (!any) return false;

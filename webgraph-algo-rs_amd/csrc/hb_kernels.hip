@@ -1144,34 +1144,48 @@ __global__ __launch_bounds__(kBlock) void batch_prep_kernel(const BatchDesc *d, 
        w += uint64_t(gridDim.x) * blockDim.x) {
     x.u.next_mod[w] = 0;
     if (x.clear_must) x.u.next_must[w] = 0;
+    if (x.mark_first) const_cast<uint32_t *>(x.u.must)[w] = 0;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void batch_compact_count_kernel(const BatchDesc *d) {
+// The batched worklist compaction runs in two phases.  Marking phase: the
+// modified nodes of the instances that mark predecessors first.  Worklist
+// phase: must | modified of the instances in list mode.
+__device__ __forceinline__ bool compact_this(const BatchDesc &x, int marking) {
+  return x.active && (marking ? x.mark_first != 0 : x.list_mode != 0);
+}
+__device__ __forceinline__ uint32_t compact_word(const BatchDesc &x, int marking, uint64_t w) {
+  return marking ? union_word(x.u.curr_mod, nullptr, w, x.n)
+                 : union_word(x.u.must, x.u.curr_mod, w, x.n);
+}
+
+__global__ __launch_bounds__(kBlock) void batch_compact_count_kernel(const BatchDesc *d,
+                                                                     int marking) {
   const BatchDesc &x = d[blockIdx.y];
-  if (!x.active || !x.list_mode) return;
+  if (!compact_this(x, marking)) return;
   const uint64_t nb = ((x.n + 31) / 32 + kBlock - 1) / kBlock;
   if (blockIdx.x >= nb) return;
   const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   uint32_t tot;
-  block_exclusive_scan(__popc(union_word(x.u.must, x.u.curr_mod, w, x.n)), &tot);
+  block_exclusive_scan(__popc(compact_word(x, marking, w)), &tot);
   if (threadIdx.x == 0) x.bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(1024) void batch_compact_scan_kernel(const BatchDesc *d) {
+__global__ __launch_bounds__(1024) void batch_compact_scan_kernel(const BatchDesc *d, int marking) {
   const BatchDesc &x = d[blockIdx.x];
-  if (!x.active || !x.list_mode) return;
+  if (!compact_this(x, marking)) return;
   const uint32_t nb = uint32_t(((x.n + 31) / 32 + kBlock - 1) / kBlock);
   scan_block_sums(x.bsum, nb, const_cast<uint32_t *>(x.u.list_len));
 }
 
-__global__ __launch_bounds__(kBlock) void batch_compact_emit_kernel(const BatchDesc *d) {
+__global__ __launch_bounds__(kBlock) void batch_compact_emit_kernel(const BatchDesc *d,
+                                                                    int marking) {
   const BatchDesc &x = d[blockIdx.y];
-  if (!x.active || !x.list_mode) return;
+  if (!compact_this(x, marking)) return;
   const uint64_t nb = ((x.n + 31) / 32 + kBlock - 1) / kBlock;
   if (blockIdx.x >= nb) return;
   const uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  uint32_t bits = union_word(x.u.must, x.u.curr_mod, w, x.n);
+  uint32_t bits = compact_word(x, marking, w);
   uint32_t tot;
   uint32_t pos = x.bsum[blockIdx.x] + block_exclusive_scan(__popc(bits), &tot);
   uint32_t *list = const_cast<uint32_t *>(x.u.list);
@@ -1179,6 +1193,24 @@ __global__ __launch_bounds__(kBlock) void batch_compact_emit_kernel(const BatchD
     const int b = __builtin_ctz(bits);
     bits &= bits - 1;
     list[pos++] = uint32_t(w * 32 + b);
+  }
+}
+
+// grid (x, instance): must |= the predecessors of the listed nodes of the
+// instances that mark them first (mark_predecessors_kernel, per instance).
+__global__ __launch_bounds__(kBlock) void batch_mark_kernel(const BatchDesc *d) {
+  const BatchDesc &x = d[blockIdx.y];
+  if (!x.active || !x.mark_first) return;
+  const uint64_t count = *x.u.list_len;
+  const uint64_t waves = uint64_t(gridDim.x) * (kBlock / 64);
+  uint32_t *must = const_cast<uint32_t *>(x.u.must);
+  for (uint64_t i = uint64_t(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64; i < count; i += waves) {
+    const uint32_t v = x.u.list[i];
+    const uint64_t e = x.u.toff[v + 1];
+    for (uint64_t k = x.u.toff[v] + __lane_id(); k < e; k += 64) {
+      const uint32_t p = x.u.tsucc[k];
+      atomicOr(must + (p >> 5), 1u << (p & 31));
+    }
   }
 }
 
@@ -1414,9 +1446,12 @@ hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
     waves.w[sh] = uint64_t(grid[sh]) * (kBlock / 64);
   }
   batch_prep_kernel<<<dim3(L.prep_blocks, L.k), kBlock, 0, s>>>(L.d, L.stats, L.work);
-  batch_compact_count_kernel<<<dim3(L.compact_blocks, L.k), kBlock, 0, s>>>(L.d);
-  batch_compact_scan_kernel<<<L.k, 1024, 0, s>>>(L.d);
-  batch_compact_emit_kernel<<<dim3(L.compact_blocks, L.k), kBlock, 0, s>>>(L.d);
+  for (int marking = 1; marking >= 0; marking--) {
+    batch_compact_count_kernel<<<dim3(L.compact_blocks, L.k), kBlock, 0, s>>>(L.d, marking);
+    batch_compact_scan_kernel<<<L.k, 1024, 0, s>>>(L.d, marking);
+    batch_compact_emit_kernel<<<dim3(L.compact_blocks, L.k), kBlock, 0, s>>>(L.d, marking);
+    if (marking) batch_mark_kernel<<<dim3(64, L.k), kBlock, 0, s>>>(L.d);
+  }
   batch_plan_kernel<<<1, 64, 0, s>>>(L.d, L.shape_inst, L.shape_off, L.stats, L.total_chunks,
                                      waves);
   for (int sh = 0; sh < kBatchShapes; sh++) {

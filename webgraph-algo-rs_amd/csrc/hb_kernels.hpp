@@ -157,7 +157,7 @@ struct BatchDesc {
   uint64_t chunk_base, chunk_end;  // in the shape's chunk space, written by the plan step
   uint64_t heavy_base, heavy_end;  // in the shape's heavy-row space
   uint32_t rounds;                 // rounds per chunk of the shape, written by the plan step
-  uint32_t pad;
+  uint32_t mark_first;             // mark the predecessors of the modified nodes into must first
 };
 
 // Queue counters of a batch: one region of kWorkCounters per shape.
