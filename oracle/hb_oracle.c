@@ -12,10 +12,13 @@
  *   or_jenkins      end to end by the five cnr-2000 golden vectors (Jenkins,
  *                   seed 42, log2m 8); a wrong step moves them far past 1e-6.
  *   or_siphash      the SipHash-2-4 test vector of the SipHash paper
- *                   (Aumasson & Bernstein 2012, appendix A) through the same
- *                   round-parameterised code.  The 1-3 zero-key output of
- *                   Rust's DefaultHasher is PARITY UNPINNED: no reference test
- *                   holds one of its values.
+ *                   (Aumasson & Bernstein 2012, appendix A), and CPython
+ *                   3.10's own SipHash-2-4 of bytes with a zero secret
+ *                   (PYTHONHASHSEED=0) on 8-byte little-endian messages: the
+ *                   zero-key, one-word path Rust's DefaultHasher takes for a
+ *                   usize, through the same round-parameterised code.  The
+ *                   1-3 round counts themselves are PARITY UNPINNED: no
+ *                   reference test and no library here holds a 1-3 value.
  *   or_get/set_register, or_add
  *                   round trips and brute force on tiny counters.
  *   or_merge_broadword

@@ -6,6 +6,7 @@ published test vectors, brute force on tiny inputs, invariants, and the
 reference's own golden vectors on cnr-2000.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -71,6 +72,31 @@ def test_siphash_published_vector(orc):
     assert orc.siphash(bytes(range(15)), k0, k1, 2, 4) == 0xA129CA6149BE45E5
     # the empty-message vector of the reference implementation's table
     assert orc.siphash(b"", k0, k1, 2, 4) == 0x726FDB47DD0E0E31
+
+
+def test_siphash_zero_key_u64_messages_match_cpython(orc):
+    """The path the reference's DefaultHasher takes for a usize: zero keys,
+    one 8-byte little-endian message, the length byte in the last block.
+    CPython 3.10 hashes bytes with SipHash-2-4 under its hash secret, which
+    PYTHONHASHSEED=0 sets to zero (hash -1 is reported as -2), so it is an
+    independent implementation of SipHash-2-4 with zero keys; the 1-3 variant
+    differs only in the round counts."""
+    import subprocess
+    import sys
+    if sys.version_info[:2] != (3, 10) or sys.hash_info.algorithm != "siphash24":
+        pytest.skip("needs CPython 3.10 with SipHash-2-4 bytes hashing")
+    xs = [0, 1, 2, 7, 255, 256, 65535, 2**32 - 1, 2**32, 10**7, 2**63, 2**64 - 1,
+          0x0123456789ABCDEF, 0xFEDCBA9876543210]
+    code = ("xs = " + repr(xs) + "\n"
+            "print(' '.join(str(hash(x.to_bytes(8, 'little')) & (2**64 - 1)) for x in xs))")
+    env = dict(os.environ, PYTHONHASHSEED="0")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                         check=True).stdout.split()
+    for x, h in zip(xs, out):
+        mine = orc.siphash(x.to_bytes(8, "little"), 0, 0, 2, 4)
+        if mine == 2**64 - 1:  # CPython reports hash -1 as -2
+            mine = 2**64 - 2
+        assert int(h) == mine, hex(x)
 
 
 def test_siphash13_is_the_round_parameterised_function(orc):
