@@ -38,7 +38,9 @@ def _compare(hbm, b, hb_batch):
     s1, s2 = hb_batch.iteration_stats(), single.iteration_stats()
     assert len(s1) == len(s2)
     for x, y in zip(s1, s2):
-        for key in ("systolic", "local", "pre_local", "visited_arcs", "modified", "nf"):
+        for key in ("systolic", "local", "pre_local", "checked", "visited_arcs", "scanned_arcs",
+                    "transpose_arcs", "copied", "modified", "gathered_slabs", "delta_gathering",
+                    "nf"):
             assert x[key] == y[key], key
 
 
