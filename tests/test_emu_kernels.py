@@ -60,3 +60,17 @@ def test_emulated_engine_matches_oracle(emu_exes, log2m, r, n, dmax, seed, heavy
     res = subprocess.run(args, capture_output=True, text=True, timeout=900)
     assert res.returncode == 0 and res.stdout.strip().endswith("OK"), res.stdout[-3000:] + res.stderr[-6000:]
     assert "DIFFERENT" not in res.stdout
+
+
+def test_emulated_count_high_registers_match_oracle(emu_exes):
+    """count() on the device path (count_all_kernel, every shape) equals the
+    oracle's or_count on crafted counters whose registers sit in the high
+    classes random graphs never reach: 28..31 for 5-bit registers (a 31
+    register adds -2^-31, the reference's i32 shift, hyper_log_log_logic.rs:
+    178) and 28..63 for 6-bit registers, mixed with zeros and uniform values."""
+    res = subprocess.run([emu_exes["oracle_count_check"], "36"], capture_output=True, text=True,
+                         timeout=900)
+    assert res.returncode == 0 and res.stdout.strip().endswith("OK"), res.stdout[-3000:] + res.stderr[-6000:]
+    assert res.stdout.count("identical") == 15
+    assert "registers up to 31: identical" in res.stdout
+    assert "registers up to 63: identical" in res.stdout
