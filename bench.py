@@ -10,6 +10,18 @@ and uploaded before the timed region.
 
 metric: arc-registers per second = sum over iterations of
 (visited arcs x registers per counter) / run time.
+
+Beside the headline line the same JSON object carries:
+  cpu_baseline  the oracle's OpenMP restatement of the reference engine on the
+                headline graph itself (init + the first iterations), with the
+                GPU timed on the same iterations;
+  batched       64 independent instances in one batch against a sequential
+                loop of single-instance runs;
+  precision     log2m 12 on the headline graph with FP32 Kahan and plain FP32
+                per-node accumulators against f64 (max / p99 relative error);
+  large         n = 10^8, about 2.1e9 arcs, log2m 6, 5-bit registers, FP32
+                Kahan accumulators, with the union step's roofline;
+  uniform_1e5   the uniform n = 10^5 graph, GPU against the oracle engine.
 """
 import argparse
 import importlib.util
@@ -53,60 +65,96 @@ def algorithmic_bytes(s, n, counter_bytes, slab_bytes, outputs, delta_aware=True
             + 16 * s["modified"] * outputs + 4 * ((n + 7) // 8))
 
 
-def _oracle_rate(orc, g, gt, log2m, reg_bits, threads, budget):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(orc, hb, g, gt, args, log2m, reg_bits):
+    """The oracle's OpenMP restatement of the reference engine
+    (hyperball_impl.rs:775-1213) on the headline graph itself: same graph
+    bytes, hash, seed, log2m, register width and outputs.  It runs init and
+    then iterations until args.cpu_budget seconds have passed (or the run
+    ends); the GPU instance then runs exactly that prefix, run(k), so both
+    rates are over identical work (the visited arcs are compared)."""
+    threads = host_threads()
     ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=log2m,
                               reg_bits=reg_bits, hash=orc.HASH_JENKINS, seed=42,
-                              sum_of_distances=True, sum_of_inverse_distances=True, threads=threads)
+                              sum_of_distances=True, sum_of_inverse_distances=True,
+                              threads=threads)
     m = 1 << log2m
     t0 = time.perf_counter()
     ref.init()
-    work = 0
+    visited = []
     while True:
         ref.iterate()
-        work += ref.visited_arcs * m
-        if ref.modified == 0 or time.perf_counter() - t0 > budget:
+        visited.append(ref.visited_arcs)
+        if ref.modified == 0 or time.perf_counter() - t0 > args.cpu_budget:
             break
-    dt = time.perf_counter() - t0
-    return work / dt, ref.iteration, dt
-
-
-def cpu_baseline(orc, hbm, args, log2m, reg_bits):
-    """The oracle's OpenMP restatement of the reference engine, on a bounded
-    sample of the same workload: the same generator at n = args.cpu_nodes
-    on the host's cores (up to 16, median of three runs), and at n / 5 on
-    one core."""
-    threads = host_threads()
-    g = hbm.Graph.web(args.cpu_nodes, 2.1, 3, 10000, 1000, 0.7, 2)
-    gt = g.transpose()
-    runs = sorted((_oracle_rate(orc, g, gt, log2m, reg_bits, threads, args.cpu_budget / 3)
-                   for _ in range(3)), key=lambda x: x[0])
-    rate, iters, dt = runs[1]  # median of three
-    g1 = hbm.Graph.web(max(1, args.cpu_nodes // 5), 2.1, 3, 10000, 1000, 0.7, 2)
-    g1t = g1.transpose()
-    rate1, iters1, dt1 = _oracle_rate(orc, g1, g1t, log2m, reg_bits, 1, args.cpu_budget / 2)
-    return {"value": rate, "unit": "arc-registers/s", "cores": threads, "kind": "port",
-            "sample": f"web graph n={g.num_nodes} arcs={g.num_arcs} (same generator), log2m={log2m}, "
-                      f"r={reg_bits}, {iters} iterations in {dt:.1f}s (median of 3 runs)",
-            "one_core": {"value": rate1, "sample": f"n={g1.num_nodes} arcs={g1.num_arcs}, "
-                                                   f"{iters1} iterations in {dt1:.1f}s"}}
+    cpu_s = time.perf_counter() - t0
+    k = len(visited)
+    work = sum(visited) * m
+    del ref
+    gpu = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        hb.run(k)
+        gpu.append(time.perf_counter() - t1)
+    gpu_s = sorted(gpu)[1]
+    same = [s["visited_arcs"] for s in hb.iteration_stats()] == visited
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    print(f"# cpu baseline: {k} iterations in {cpu_s:.1f}s on {threads} threads; "
+          f"GPU {gpu_s * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    return {"value": work / cpu_s, "unit": "arc-registers/s", "cores": threads, "kind": "port",
+            "sample": f"the headline graph itself (n={g.num_nodes}, arcs={g.num_arcs}, log2m "
+                      f"{log2m}, r {reg_bits}, jenkins(42), sum of distances + harmonic): init + "
+                      f"the first {k} iterations in {cpu_s:.1f}s on {threads} threads",
+            "cpu_model": cpu_model(), "host_cores_visible": os.cpu_count(),
+            "affinity_cores": affinity,
+            "threads_note": "16 threads: the host CPU share of one GPU on the measurement box",
+            "gpu_same_work": {"value": work / gpu_s, "ms": gpu_s * 1e3, "iterations": k,
+                              "same_visited_arcs": same},
+            "gpu_over_cpu": cpu_s / gpu_s}
 
 
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "union_step_web1e7_log2m10_r6_pmc.md")
+KERNEL_SOURCE = os.path.join(PKG, "csrc", "hb_kernels.hip")
+
+
+def kernel_source_sha():
+    import hashlib
+    with open(KERNEL_SOURCE, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
 
 
 def profiled_traffic(args):
     """HBM bytes per union step from the committed FETCH_SIZE / WRITE_SIZE
-    passes (copy-calibrated), for the configuration they were taken on."""
+    passes (copy-calibrated), for the configuration they were taken on, and
+    only when they were taken on the kernel source that is built now."""
     if (args.nodes, args.log2m, args.reg_bits) != (10_000_000, 10, 6):
-        return None
+        return None, "no profile for this configuration"
+    value, sha = None, None
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_union_step:"):
-                    return float(line.split(":")[1])
+                    value = float(line.split(":")[1])
+                if line.startswith("kernel_source_sha256:"):
+                    sha = line.split(":")[1].strip()
     except OSError:
-        pass
-    return None
+        return None, "profile missing"
+    if sha != kernel_source_sha():
+        return None, "the profile was taken on other kernel sources"
+    return value, os.path.relpath(TRAFFIC_PROFILE, REPO) + f" (hb_kernels.hip sha256 {sha[:16]})"
 
 
 def batch_workload(hbm, args):
@@ -187,6 +235,8 @@ def bench_batch(hbm, args):
 
 
 def host_threads():
+    """Host threads for the CPU legs: the cores this process may run on, at
+    most 16 (the host CPU share of one GPU on the measurement box)."""
     threads = os.cpu_count() or 1
     try:
         threads = len(os.sched_getaffinity(0))
@@ -313,55 +363,77 @@ def bench_host_counters(hbm, args):
     return out
 
 
-def bench_large(hbm, args):
+def bench_large(hbm, args, compare_transposes=True):
     """One large instance: n = 10^8 nodes, the same generator with out-degree
     Zipf(2.1) on [4, 10000] and a +-10,000 window (about 2e9 arcs), log2m 6,
     5-bit registers, systolic frontier, f32 Kahan-compensated accumulators.
-    The transpose is built on the device inside build(); the host counting
-    sort and hb_graph_transpose_device are timed beside it and compared."""
+    The transpose is built on the device inside build(); with
+    compare_transposes the host counting sort and hb_graph_transpose_device
+    are timed beside it and compared.  The union step's roofline uses the
+    same algorithmic-bytes model as the headline (40-byte counters)."""
     t = time.perf_counter()
     g = hbm.Graph.web(args.large_nodes, 2.1, 4, 10000, 10000, 0.7, 5)
     print(f"# graph n={g.num_nodes} arcs={g.num_arcs} built in {time.perf_counter() - t:.1f}s",
           file=sys.stderr, flush=True)
-    t = time.perf_counter()
-    gt = g.transpose()
-    host_t = time.perf_counter() - t
-    t = time.perf_counter()
-    gtd = g.transpose(device=0)
-    dev_t = time.perf_counter() - t
-    same = bool(np.array_equal(gt.offsets, gtd.offsets) and np.array_equal(gt.successors, gtd.successors))
-    print(f"# transpose: host {host_t:.1f}s, device (with both copies) {dev_t:.1f}s, "
-          f"identical {same}", file=sys.stderr, flush=True)
-    del gt, gtd
+    setup = {}
+    if compare_transposes:
+        t = time.perf_counter()
+        gt = g.transpose()
+        host_t = time.perf_counter() - t
+        t = time.perf_counter()
+        gtd = g.transpose(device=0)
+        dev_t = time.perf_counter() - t
+        same = bool(np.array_equal(gt.offsets, gtd.offsets) and
+                    np.array_equal(gt.successors, gtd.successors))
+        print(f"# transpose: host {host_t:.1f}s, device (with both copies) {dev_t:.1f}s, "
+              f"identical {same}", file=sys.stderr, flush=True)
+        del gt, gtd
+        setup = {"host_transpose_s": host_t, "device_transpose_with_copies_s": dev_t,
+                 "transpose_identical": same}
     t = time.perf_counter()
     hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, None, 6).transpose_on_device()
           .register_bits(5).jenkins(42).sum_of_distances(True).sum_of_inverse_distances(True)
           .accumulator_precision(hbm.ACC_F32_KAHAN).build())
     hb.set_timing(True)
-    build_t = time.perf_counter() - t
-    print(f"# instance built (upload + device transpose) in {build_t:.1f}s", file=sys.stderr, flush=True)
+    setup["build_with_device_transpose_s"] = time.perf_counter() - t
+    print(f"# instance built (upload + device transpose) in {setup['build_with_device_transpose_s']:.1f}s",
+          file=sys.stderr, flush=True)
     nodes, arcs = g.num_nodes, g.num_arcs
     del g
+    p = hb.params()
+    C = p["words_per_counter"] * 8
     for i in range(args.warmup):
         hb.run_until_done()
         print(f"# warmup {i} done", file=sys.stderr, flush=True)
-    work, union_ms, iters = 0, 0.0, 0
+    work, union_ms, iters, nbytes, dense_bytes, dense_ms = 0, 0.0, 0, 0.0, 0.0, 0.0
+    steps = max(1, min(args.steps, args.large_steps))
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         hb.run_until_done()
         st = hb.iteration_stats()
         iters = len(st)
-        work += sum(s["visited_arcs"] for s in st) * 64
-        union_ms += sum(s["union_ms"] for s in st)
+        for s_ in st:
+            work += s_["visited_arcs"] * 64
+            union_ms += s_["union_ms"]
+            b = algorithmic_bytes(s_, nodes, C, 4 * 5, 2)
+            nbytes += b
+            if not s_["systolic"]:
+                dense_bytes += b
+                dense_ms += s_["union_ms"]
         print(f"# step {i}: {iters} iterations", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     hb.close()
+    achieved = nbytes / (union_ms * 1e-3) / 1e9 if union_ms > 0 else 0.0
     return {"workload": f"web graph n={nodes} arcs={arcs} (+-10000 window, out-degree Zipf(2.1) "
                         "on [4, 10000]), log2m 6, r 5, f32 Kahan accumulators",
-            "value": work / elapsed, "unit": "arc-registers/s", "ms_per_step": elapsed * 1e3 / args.steps,
-            "iterations": iters, "union_ms_per_step": union_ms / args.steps,
-            "setup": {"host_transpose_s": host_t, "device_transpose_with_copies_s": dev_t,
-                      "transpose_identical": same, "build_with_device_transpose_s": build_t}}
+            "value": work / elapsed, "unit": "arc-registers/s", "ms_per_step": elapsed * 1e3 / steps,
+            "steps": steps, "iterations": iters, "union_ms_per_step": union_ms / steps,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": achieved / PEAK_HBM_GBS,
+                         "dense_iterations_frac": (dense_bytes / (dense_ms * 1e-3) / 1e9 / PEAK_HBM_GBS
+                                                   if dense_ms > 0 else None),
+                         "kernel": "union step", "bytes_model": "as the headline's, C = 40 bytes"},
+            "setup": setup}
 
 
 def main():
@@ -372,8 +444,8 @@ def main():
     ap.add_argument("--nodes", type=int, default=10_000_000)
     ap.add_argument("--log2m", type=int, default=10)
     ap.add_argument("--reg-bits", type=int, default=6)
-    ap.add_argument("--cpu-nodes", type=int, default=1_000_000)
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of CPU iterations on the headline graph (whole iterations)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=64, help="instances in the batched leg (0 = skip)")
     ap.add_argument("--only-batch", action="store_true")
@@ -386,6 +458,9 @@ def main():
     ap.add_argument("--large", action="store_true",
                     help="only the large instance (n = --large-nodes, log2m 6, r 5)")
     ap.add_argument("--large-nodes", type=int, default=100_000_000)
+    ap.add_argument("--large-steps", type=int, default=3, help="timed runs of the large instance")
+    ap.add_argument("--no-large", action="store_true", help="skip the large instance leg")
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision leg")
     ap.add_argument("--host-counters", action="store_true",
                     help="only the counters-in-host-memory comparison (n = --host-nodes)")
     ap.add_argument("--host-nodes", type=int, default=1_000_000)
@@ -486,6 +561,7 @@ def main():
             print("# " + json.dumps(s), file=sys.stderr)
 
     achieved = bytes_total / (union_ms * 1e-3) / 1e9 if union_ms > 0 else 0.0
+    traffic, traffic_src = profiled_traffic(args)
     result = {
         "metric": "HyperBall arc-registers/s (visited arcs x registers per counter, run to stabilisation)",
         "value": work / elapsed,
@@ -506,7 +582,7 @@ def main():
                    "heavy_row_threshold": chosen,
                    "outputs": ["neighborhood_function", "sum_of_distances", "harmonic_centrality"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": profiled_traffic(args),
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "union step (heavy_kernel + union_kernel, one launch each per "
                                "iteration)", "launches": launches,
                      "avg_launch_ms": union_ms / max(1, launches),
@@ -518,14 +594,18 @@ def main():
                      "no_delta_model_frac": (bytes_no_delta / (union_ms * 1e-3) / 1e9 / PEAK_HBM_GBS
                                              if union_ms > 0 else None)},
     }
+    orc = None
+    if not args.no_cpu_baseline:
+        orc = _load("hb_oracle_py", os.path.join(REPO, "oracle", "oracle.py"))
+        result["cpu_baseline"] = cpu_baseline(orc, hb, g, gt, args, args.log2m, args.reg_bits)
     hb.close()
     del g, gt
     if args.batch:
         result["batched"] = bench_batch(hbm, args)
-    orc = None
-    if not args.no_cpu_baseline:
-        orc = _load("hb_oracle_py", os.path.join(REPO, "oracle", "oracle.py"))
-        result["cpu_baseline"] = cpu_baseline(orc, hbm, args, args.log2m, args.reg_bits)
+    if not args.no_precision:
+        result["precision"] = bench_precision(hbm, args)
+    if not args.no_large:
+        result["large"] = bench_large(hbm, args, compare_transposes=False)
     if not args.no_small:
         result["uniform_1e5"] = small_uniform(hbm, orc, args)
     print(json.dumps(result), flush=True)
