@@ -583,8 +583,8 @@ def main():
                    "outputs": ["neighborhood_function", "sum_of_distances", "harmonic_centrality"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "union step (heavy_kernel + union_kernel, one launch each per "
-                               "iteration)", "launches": launches,
+                     "kernel": "union step (union_kernel: heavy-row phase, then chunks; one "
+                               "launch per iteration)", "launches": launches,
                      "avg_launch_ms": union_ms / max(1, launches),
                      "algorithmic_bytes_per_launch": bytes_total / max(1, launches),
                      "dense_iterations_frac": (dense_bytes / (dense_ms * 1e-3) / 1e9 / PEAK_HBM_GBS

@@ -96,12 +96,12 @@ struct Instance {
   bool weighted = false, has_t = false, do_sd = false, do_hc = false;
   uint32_t acc_mode = ACC_F64;
   std::vector<Discount> discounts;
-  // Rows longer than heavy_degree go to the workgroup-per-node kernel.
+  // Rows longer than heavy_degree take a whole workgroup each (the union
+  // kernel's heavy-row phase).
   // heavy_nodes holds every node with more than kHeavyFloor successors,
   // sorted by decreasing degree, so a threshold selects a prefix.
   static constexpr uint64_t kHeavyFloor = 64;
   static constexpr uint64_t kDefaultHeavy = 512;
-  static constexpr uint32_t kHeavyGridCap = 4096;
   uint64_t heavy_degree = kDefaultHeavy;
   std::vector<uint32_t> heavy_nodes;
   std::vector<uint64_t> heavy_degs;
@@ -275,7 +275,7 @@ struct Instance {
     bool list_mode;
     bool mark_first;  // mark the predecessors of the modified nodes before the union
     double base_nf;
-    uint32_t grid, heavy_grid;
+    uint32_t grid;
   };
 
   Plan plan_iteration() {
@@ -375,7 +375,6 @@ struct Instance {
     if (!p.list_mode)
       p.grid = uint32_t(std::min<uint64_t>(max_grid, std::max<uint64_t>(8, (n + npb - 1) / npb)));
     p.grid = (p.grid + 7) / 8 * 8;
-    p.heavy_grid = uint32_t(std::min<uint64_t>(heavy_count, kHeavyGridCap));
     return p;
   }
 
@@ -430,7 +429,7 @@ struct Instance {
     if (p.list_mode)
       HB_HIP(launch_compact(p.ua.must, p.ua.curr_mod, n, bsum.p, list.p, list_len.p, stream));
     if (timing) HB_HIP(hipEventRecord(ev[1], stream));
-    HB_HIP(launch_union(int(r), int(log2m), p.ua, p.grid, p.heavy_grid, stream));
+    HB_HIP(launch_union(int(r), int(log2m), p.ua, p.grid, stream));
     if (timing) HB_HIP(hipEventRecord(ev[2], stream));
     HB_HIP(launch_reduce_stats(stats.p, stat_slots.p, h_stats, stream));  // h_stats: pinned
     if (timing) HB_HIP(hipEventRecord(ev[3], stream));
@@ -594,7 +593,7 @@ struct Batch {
 
   std::vector<uint32_t> h_shape_inst;
 
-  // Heavy-row ranges, shape by shape (the heavy kernels' launch sizes are
+  // Heavy-row ranges, shape by shape (each shape's total is a kernel argument
   // captured in the graph: a change drops it).
   void refresh_heavy() {
     for (int sh = 0; sh < kBatchShapes; sh++) {
@@ -605,10 +604,8 @@ struct Batch {
         acc += inst[i]->heavy_count;
         h_desc[i].heavy_end = acc;
       }
-      const uint32_t hg = uint32_t(std::min<uint64_t>(acc, Instance::kHeavyGridCap));
-      if (hg != L.heavy_grid[sh] || acc != L.total_heavy[sh]) drop_graph();
+      if (acc != L.total_heavy[sh]) drop_graph();
       L.total_heavy[sh] = acc;
-      L.heavy_grid[sh] = hg;
     }
   }
 

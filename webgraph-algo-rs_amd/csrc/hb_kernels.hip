@@ -696,47 +696,16 @@ __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, ui
   lane_sync<64>();  // the next chunk's staging overwrites meta
 }
 
-// Shapes of 5 dwords per lane fit four wavefronts per SIMD (at most 128
-// VGPRs); the hint keeps the register allocator there.
-template <int R, int G, int SPL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R * SPL <= 5 ? 4 : 1)))
-void union_kernel(UnionArgs a) {
-  const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
-  const uint32_t rounds = __builtin_amdgcn_readfirstlane(
-      chunk_rounds(count, 64 / G, uint64_t(gridDim.x) * (kBlock / 64)));
-  const uint64_t cp = uint64_t(rounds) * (64 / G);
-  const uint64_t chunks = (count + cp - 1) / cp;
-  constexpr int kWaveIds = 2 * 64 * ids_per_lane<G>();
-  constexpr int kWaveMeta = int(chunk_positions<G>());
-  __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
-  __shared__ NodeMeta s_meta[(kBlock / 64) * kWaveMeta];
-  uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
-  NodeMeta *meta = s_meta + (threadIdx.x / 64) * kWaveMeta;
-  Acc acc;
-  // the loop body stays single: a first-iteration flag inside the loop gets
-  // it peeled and costs 18 VGPRs (an occupancy step)
-  uint64_t c = 0;
-  if (__lane_id() == 0) c = first_chunk(a.work, chunks);
-  c = __shfl(c, 0);
-  while (c != UINT64_MAX) {
-    run_chunk<R, G, SPL>(a, c, count, rounds, wave_ids, meta, acc);
-    if (__lane_id() == 0) c = grab_chunk(a.work, chunks);
-    c = __shfl(c, 0);
-  }
-  flush_stats(acc, a.stats, a.stat_slots);
-}
-
 // Rows longer than heavy_degree: the whole workgroup works on one node.  Its
-// kBlock/G lane groups take interleaved G-wide slices of the row, keep
-// private maxima, and fold them in LDS; lane group 0 finishes the node.  Every
-// thread of the block calls it with the same v.  s_acc holds kBlock * D words.
-// DELTA_OK: whether the node may use delta gathering (when the iteration's
-// flags ask for it); the batched kernel keeps it off to stay within two
-// wavefronts per SIMD.
-template <int R, int G, int SPL, bool DELTA_OK = true>
+// kBlock/G lane groups take interleaved slices of the row, keep private
+// maxima, and fold them in LDS; lane group 0 finishes the node.  Every thread
+// of the block calls it with the same v.  LDS: s_acc holds kBlock * D words,
+// s_ids 2 * kBlock * ids_per_lane<G>() words (a slot per lane group, as in the
+// chunk phase), s_sum two words (any lane group loaded the counter; visited
+// arcs).
+template <int R, int G, int SPL>
 __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc,
-                                           uint32_t *s_ids, uint32_t *s_visit, uint32_t *s_any,
+                                           uint32_t *s_ids, uint32_t *s_sum,
                                            Acc &acc) {
   constexpr int D = R * SPL, NG = kBlock / G, CD = G * D;
   const int g = threadIdx.x / G, gl = threadIdx.x % G;
@@ -752,21 +721,14 @@ __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint3
     const uint64_t first = beg + uint64_t(g) * slice_width<G>();
     const uint64_t stride = uint64_t(NG) * slice_width<G>();
     uint32_t *slot = s_ids + g * 2 * slice_width<G>();
-    if (DELTA_OK && (a.flags & F_SEEDED))
-      merge_slices<R, G, SPL, 2>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                                 gathered, slot);
-    else if (DELTA_OK && (a.flags & F_DELTA))
-      merge_slices<R, G, SPL, 1>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                                 gathered, slot);
-    else
-      merge_slices<R, G, SPL, 0>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                                 gathered, slot);
+    merge_successors<R, G, SPL>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                gathered, slot);
   }
   acc.gathered += gathered;
-  if (threadIdx.x == 0) *s_any = 0;
+  if (threadIdx.x == 0) s_sum[0] = s_sum[1] = 0;
   __syncthreads();
-  if (loaded && gl == 0) atomicOr(s_any, 1u);
-  if (gl == 0) s_visit[g] = nvisit;
+  if (loaded && gl == 0) atomicOr(s_sum, 1u);
+  if (gl == 0 && nvisit) atomicAdd(s_sum + 1, nvisit);
 #pragma unroll
   for (int k = 0; k < D; k++) s_acc[g * CD + gl * D + k] = loaded ? cur[k] : 0u;
   __syncthreads();
@@ -786,9 +748,8 @@ __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint3
     __syncthreads();
   }
   if (g == 0) {
-    const bool any = *s_any != 0;
-    uint32_t total_visit = 0;
-    for (int k = 0; k < NG; k++) total_visit += s_visit[k];
+    const bool any = s_sum[0] != 0;
+    const uint32_t total_visit = s_sum[1];
     if (any) {
       load_part<D>(prev, a.curr + uint64_t(v) * counter_stride(CD) + gl * D);
 #pragma unroll
@@ -803,21 +764,68 @@ __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint3
   __syncthreads();
 }
 
+// LDS of a union kernel's workgroup.  The heavy-row phase and the chunk
+// phase share the successor-id slots; the heavy phase's fold buffer overlays
+// the chunk phase's node data (a workgroup leaves the heavy phase at a
+// barrier).
 template <int R, int G, int SPL>
-__global__ __launch_bounds__(kBlock) void heavy_kernel(UnionArgs a) {
-  __shared__ uint32_t s_acc[kBlock * R * SPL];
-  __shared__ uint32_t s_ids[2 * kBlock * ids_per_lane<G>()];
-  __shared__ uint32_t s_visit[kBlock];
-  __shared__ uint32_t s_any;
-  __shared__ uint64_t s_h;
-  Acc acc;
+struct UnionLds {
+  static constexpr int kWaveIds = 2 * 64 * ids_per_lane<G>();
+  static constexpr int kWaveMeta = int(chunk_positions<G>());
+  uint32_t ids[(kBlock / 64) * kWaveIds];
+  union {
+    NodeMeta meta[(kBlock / 64) * kWaveMeta];
+    struct {
+      uint32_t acc[kBlock * R * SPL];
+      uint32_t sum[2];
+      uint64_t h;  // the heavy row being worked on
+    } heavy;
+  } u;
+};
+
+// Work counter [8] of a.work hands out the heavy rows.  Every workgroup takes
+// rows until none is left (the heaviest first: the list is sorted by
+// decreasing degree), then turns to the chunk queues.  Returns when no heavy
+// row is left; uniform over the workgroup.
+template <int R, int G, int SPL>
+__device__ __forceinline__ void heavy_phase(const UnionArgs &a, UnionLds<R, G, SPL> &L, Acc &acc) {
   for (;;) {
-    if (threadIdx.x == 0) s_h = atomicAdd(a.work + 8 * kWorkStride, 1u);
+    if (threadIdx.x == 0) L.u.heavy.h = atomicAdd(a.work + 8 * kWorkStride, 1u);
     __syncthreads();
-    const uint64_t h = s_h;
+    const uint64_t h = L.u.heavy.h;
     __syncthreads();
     if (h >= a.heavy_count) break;
-    heavy_node<R, G, SPL>(a, a.heavy_list[h], s_acc, s_ids, s_visit, &s_any, acc);
+    heavy_node<R, G, SPL>(a, a.heavy_list[h], L.u.heavy.acc, L.ids, L.u.heavy.sum, acc);
+  }
+}
+
+// The union step of one iteration: first the rows longer than heavy_degree,
+// a workgroup per row (heavy_phase), then chunks of the other nodes, a lane
+// group per node.  Shapes of 5 dwords per lane fit four wavefronts per SIMD
+// (at most 128 VGPRs); the hint keeps the register allocator there.
+template <int R, int G, int SPL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R * SPL <= 5 ? 4 : 1)))
+void union_kernel(UnionArgs a) {
+  __shared__ UnionLds<R, G, SPL> L;
+  Acc acc;
+  if (a.heavy_count) heavy_phase<R, G, SPL>(a, L, acc);
+  const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
+  const uint32_t rounds = __builtin_amdgcn_readfirstlane(
+      chunk_rounds(count, 64 / G, uint64_t(gridDim.x) * (kBlock / 64)));
+  const uint64_t cp = uint64_t(rounds) * (64 / G);
+  const uint64_t chunks = (count + cp - 1) / cp;
+  uint32_t *wave_ids = L.ids + (threadIdx.x / 64) * UnionLds<R, G, SPL>::kWaveIds;
+  NodeMeta *meta = L.u.meta + (threadIdx.x / 64) * UnionLds<R, G, SPL>::kWaveMeta;
+  // the loop body stays single: a first-iteration flag inside the loop gets
+  // it peeled and costs 18 VGPRs (an occupancy step)
+  uint64_t c = 0;
+  if (__lane_id() == 0) c = first_chunk(a.work, chunks);
+  c = __shfl(c, 0);
+  while (c != UINT64_MAX) {
+    run_chunk<R, G, SPL>(a, c, count, rounds, wave_ids, meta, acc);
+    if (__lane_id() == 0) c = grab_chunk(a.work, chunks);
+    c = __shfl(c, 0);
   }
   flush_stats(acc, a.stats, a.stat_slots);
 }
@@ -1259,24 +1267,41 @@ __global__ void batch_plan_kernel(BatchDesc *d, const uint32_t *shape_inst,
   }
 }
 
-// The union step of one batched shape: per-XCD queues over the shape's chunk
-// space; a wavefront finds the instance of its chunk and runs it.
-// The hint keeps every shape at three wavefronts per SIMD (at most 168 VGPRs,
-// no spills); left alone, log2m 10 with 6-bit registers takes 175.
+// The union step of one batched shape: first the shape's heavy rows, a
+// workgroup per row, from the shape's heavy-row space (prefix sums over its
+// instances); then per-XCD queues over the shape's chunk space, a wavefront
+// finding the instance of its chunk.  Statistics go to the instance they
+// belong to.  The hint keeps every shape at three wavefronts per SIMD (at
+// most 168 VGPRs, no spills); left alone, log2m 10 with 6-bit registers takes
+// 175.
 template <int R, int G, int SPL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
-void batch_union_kernel(const BatchDesc *d, const uint32_t *inst,
-                                                             int cnt, const uint64_t *total_chunks,
-                                                             uint32_t *work) {
-  constexpr int kWaveIds = 2 * 64 * ids_per_lane<G>();
-  constexpr int kWaveMeta = int(chunk_positions<G>());
-  __shared__ uint32_t s_ids[(kBlock / 64) * kWaveIds];
-  __shared__ NodeMeta s_meta[(kBlock / 64) * kWaveMeta];
-  uint32_t *wave_ids = s_ids + (threadIdx.x / 64) * kWaveIds;
-  NodeMeta *meta = s_meta + (threadIdx.x / 64) * kWaveMeta;
-  const uint64_t chunks = *total_chunks;
+void batch_union_kernel(const BatchDesc *d, const uint32_t *inst, int cnt,
+                        const uint64_t *total_chunks, uint64_t total_heavy, uint32_t *work) {
+  __shared__ UnionLds<R, G, SPL> L;
   Acc acc;
   int64_t current = -1;
+  for (;;) {
+    if (!total_heavy) break;
+    if (threadIdx.x == 0) L.u.heavy.h = atomicAdd(work + 8 * kWorkStride, 1u);
+    __syncthreads();
+    const uint64_t h = L.u.heavy.h;
+    __syncthreads();
+    if (h >= total_heavy) break;
+    const uint32_t i = find_in_shape(d, inst, cnt, h, true);
+    const BatchDesc &x = d[i];
+    if (!x.active) continue;
+    if (int64_t(i) != current) {
+      if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
+      acc = Acc();
+      current = i;
+    }
+    heavy_node<R, G, SPL>(x.u, x.u.heavy_list[h - x.heavy_base], L.u.heavy.acc, L.ids,
+                          L.u.heavy.sum, acc);
+  }
+  uint32_t *wave_ids = L.ids + (threadIdx.x / 64) * UnionLds<R, G, SPL>::kWaveIds;
+  NodeMeta *meta = L.u.meta + (threadIdx.x / 64) * UnionLds<R, G, SPL>::kWaveMeta;
+  const uint64_t chunks = *total_chunks;
   uint64_t c = 0;
   if (__lane_id() == 0) c = first_chunk(work, chunks);
   c = __shfl(c, 0);
@@ -1291,38 +1316,6 @@ void batch_union_kernel(const BatchDesc *d, const uint32_t *inst,
     const uint64_t count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
     run_chunk<R, G, SPL>(x.u, c - x.chunk_base, count, x.rounds, wave_ids, meta, acc);
     if (__lane_id() == 0) c = grab_chunk(work, chunks);
-  }
-  if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
-}
-
-// The heavy rows of one batched shape, one workgroup per row.
-template <int R, int G, int SPL>
-__global__ __launch_bounds__(kBlock) void batch_heavy_kernel(const BatchDesc *d, const uint32_t *inst,
-                                                             int cnt, uint64_t total_heavy,
-                                                             uint32_t *work) {
-  __shared__ uint32_t s_acc[kBlock * R * SPL];
-  __shared__ uint32_t s_ids[2 * kBlock * ids_per_lane<G>()];
-  __shared__ uint32_t s_visit[kBlock];
-  __shared__ uint32_t s_any;
-  __shared__ uint64_t s_h;
-  Acc acc;
-  int64_t current = -1;
-  for (;;) {
-    if (threadIdx.x == 0) s_h = atomicAdd(work + 8 * kWorkStride, 1u);
-    __syncthreads();
-    const uint64_t h = s_h;
-    __syncthreads();
-    if (h >= total_heavy) break;
-    const uint32_t i = find_in_shape(d, inst, cnt, h, true);
-    const BatchDesc &x = d[i];
-    if (!x.active) continue;
-    if (int64_t(i) != current) {
-      if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
-      acc = Acc();
-      current = i;
-    }
-    heavy_node<R, G, SPL>(x.u, x.u.heavy_list[h - x.heavy_base], s_acc, s_ids, s_visit, &s_any,
-                          acc);
   }
   if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
 }
@@ -1363,11 +1356,10 @@ uint32_t resident_grid(K kernel, std::atomic<uint32_t> *cache) {
 }
 
 template <int R, int G, int SPL>
-hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, uint32_t heavy_grid, hipStream_t s) {
+hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, hipStream_t s) {
   static std::atomic<uint32_t> cache[kMaxDevices];
   const uint32_t cap = resident_grid(union_kernel<R, G, SPL>, cache);
   if (cap && grid > cap) grid = cap;
-  if (heavy_grid) heavy_kernel<R, G, SPL><<<heavy_grid, kBlock, 0, s>>>(a);
   union_kernel<R, G, SPL><<<grid, kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
@@ -1407,10 +1399,9 @@ bool union_shape(int r, int log2m, int *lanes, int *spl) {
   return false;
 }
 
-hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, uint32_t heavy_grid,
-                        hipStream_t s) {
+hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s) {
 #define X(RR, LM, GG, SS) \
-  if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS>(a, grid, heavy_grid, s);
+  if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS>(a, grid, s);
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
@@ -1462,12 +1453,9 @@ hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
     switch (sh) {
 #define X(ID, RR, GG, SS)                                                                      \
   case ID:                                                                                     \
-    if (L.heavy_grid[sh])                                                                      \
-      batch_heavy_kernel<RR, GG, SS><<<L.heavy_grid[sh], kBlock, 0, s>>>(L.d, inst, cnt,      \
-                                                                          L.total_heavy[sh],   \
-                                                                          work);               \
     batch_union_kernel<RR, GG, SS><<<grid[sh], kBlock, 0, s>>>(L.d, inst, cnt,                \
-                                                                L.total_chunks + sh, work);     \
+                                                                L.total_chunks + sh,           \
+                                                                L.total_heavy[sh], work);      \
     break;
       HB_BATCH_SHAPES(X)
 #undef X

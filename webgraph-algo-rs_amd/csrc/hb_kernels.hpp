@@ -113,7 +113,7 @@ struct UnionArgs {
   uint64_t *stat_slots;    // kStatSlots * kStatSlotWords, zeroed before the launch; null:
                            // wavefronts add to *stats directly
   uint32_t *work;          // kWorkCounters * kWorkStride words, zeroed before the launch
-  uint64_t heavy_degree;         // rows longer than this go to heavy_kernel
+  uint64_t heavy_degree;         // rows longer than this take a workgroup each (heavy_phase)
   const uint32_t *heavy_list;    // the nodes with such rows
   uint64_t heavy_count;
   // Change masks: bit (slab >> chg_shift) of chg_cur[s] is set when a
@@ -177,7 +177,6 @@ struct BatchLaunch {
   const uint32_t *shape_off;  // kBatchShapes + 1
   uint32_t first[kBatchShapes], count[kBatchShapes];
   uint32_t prep_blocks, compact_blocks, union_grid;
-  uint32_t heavy_grid[kBatchShapes];
   uint64_t total_heavy[kBatchShapes];
 };
 
@@ -190,10 +189,9 @@ hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s);
 // lane.  Returns false for shapes without a kernel.
 bool union_shape(int r, int log2m, int *lanes, int *slabs_per_lane);
 
-// Launches heavy_kernel over a.heavy_list (heavy_grid workgroups, 0 = none)
-// and then union_kernel (grid workgroups).
-hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, uint32_t heavy_grid,
-                        hipStream_t s);
+// Launches union_kernel (grid workgroups, capped at the resident ones): the
+// heavy rows of a.heavy_list first, a workgroup each, then the chunks.
+hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_count_all(int r, int log2m, const uint32_t *counters, uint64_t n,
                             double alpha_m_m, uint32_t m, const double *lc_table, double *out,
                             hipStream_t s);
