@@ -213,6 +213,18 @@ hb_status hb_get_params(hb_instance *hb, hb_params *out);
  * j at bit j*r, LSB first.  which = 0: current state, 1: the other buffer
  * (all zero before the first iteration). */
 hb_status hb_get_counters(hb_instance *hb, int which, uint64_t *out, size_t words);
+/* hb_get_counters that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * conversion to the reference layout goes through a 16 MiB staging array in
+ * device memory that the library holds per GPU (from the first call on that
+ * GPU until the process ends) instead of a temporary hipMalloc; counters are
+ * converted and copied in chunks of at most 16 MiB, so every instance size is
+ * accepted (a single counter larger than 16 MiB would be rejected with
+ * HB_E_INVALID_ARG).  All calls on one GPU share that array: calls that
+ * overlap on one GPU (from several threads) race and must be serialised by
+ * the caller.  Same arguments, checks and results as hb_get_counters. */
+hb_status hb_get_counters_noalloc(hb_instance *hb, int which, uint64_t *out, size_t words);
 /* Modified-counter bits of the last iteration (AtomicBitVec layout: bit v of
  * word v/64). */
 hb_status hb_get_modified(hb_instance *hb, uint64_t *out, size_t words);

@@ -33,6 +33,9 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+// device arrays are host arrays here: a symbol's address is its own
+#define HIP_SYMBOL(x) (&(x))
 #define __HIP_MEMORY_SCOPE_AGENT 0
 #define __hip_atomic_load(p, order, scope) __atomic_load_n((p), (order))
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
@@ -101,6 +104,10 @@ typedef struct {
 } hipDeviceProp_t;
 
 hipError_t hipMalloc(void **p, size_t bytes);
+inline hipError_t hipGetSymbolAddress(void **p, const void *symbol) {
+  *p = const_cast<void *>(symbol);
+  return hipSuccess;
+}
 hipError_t hipFree(void *p);
 hipError_t hipHostMalloc(void **p, size_t bytes, unsigned flags);
 hipError_t hipHostFree(void *p);

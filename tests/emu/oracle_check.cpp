@@ -59,7 +59,7 @@ int main(int argc, char **argv) {
   hb_params p;
   check(hb_get_params(hb, &p), "hb_get_params");
   const size_t words = size_t(n) * p.words_per_counter;
-  std::vector<uint64_t> c(words), mod((n + 63) / 64);
+  std::vector<uint64_t> c(words), c2(words), mod((n + 63) / 64);
   std::printf("n=%llu arcs=%llu log2m=%u r=%u heavy=%u\n", (unsigned long long)n,
               (unsigned long long)gg.num_arcs, log2m, r, heavy);
   check(hb_init(hb), "hb_init");
@@ -69,6 +69,13 @@ int main(int argc, char **argv) {
     check(hb_iterate(hb), "hb_iterate");
     or_hb_iterate(ref);
     check(hb_get_counters(hb, 0, c.data(), words), "hb_get_counters");
+    check(hb_get_counters_noalloc(hb, it & 1, c2.data(), words), "hb_get_counters_noalloc");
+    std::vector<uint64_t> c1(words);
+    check(hb_get_counters(hb, it & 1, c1.data(), words), "hb_get_counters");
+    if (std::memcmp(c1.data(), c2.data(), words * 8) != 0) {
+      std::printf("hb_get_counters_noalloc DIFFERENT at iteration %d\n", it);
+      bad++;
+    }
     check(hb_get_modified(hb, mod.data(), mod.size()), "hb_get_modified");
     hb_iteration_stats st;
     check(hb_get_iteration_stats(hb, uint64_t(it), &st), "hb_get_iteration_stats");

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -912,6 +913,33 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     return inst;
 }
 
+// Staging memory of hb_get_counters_noalloc: a file-scope device array, so
+// that entry point allocates nothing.  Each device holds its own copy from the
+// first use until the process ends.  Host code uses only the address that
+// hipGetSymbolAddress returns for the current device (scratch_address), never
+// the array's name.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchBytes / 8];
+
+constexpr int kScratchDevices = 64;
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device.
+uint64_t *scratch_address(int device) {
+  static std::mutex mu;
+  static uint64_t *addr[kScratchDevices] = {};
+  if (device < 0 || device >= kScratchDevices)
+    fail(HB_E_NO_DEVICE, "device " + std::to_string(device) + " is beyond the scratch table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void *p = nullptr;
+    HB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(HB_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    addr[device] = static_cast<uint64_t *>(p);
+  }
+  return addr[device];
+}
+
 }  // namespace hb
 
 using namespace hb;
@@ -1103,27 +1131,48 @@ hb_status hb_get_params(hb_instance *hb, hb_params *out) {
   });
 }
 
+// Counters in the reference layout, converted on the device in chunks of at
+// most `stage_words` words through `stage` and copied to `out`.
+static void copy_counters(Instance &I, int which, uint64_t *out, size_t words, uint64_t *stage,
+                          uint64_t stage_words) {
+  if (!out) fail(HB_E_INVALID_ARG, "null output");
+  if (!I.inited) fail(HB_E_NOT_RUN, "counters are not initialised; call hb_init or hb_run");
+  if (words < I.n * I.W) fail(HB_E_INVALID_ARG, "output buffer too small");
+  if (which && I.iteration == 0) {  // the other buffer before the first iteration
+    std::memset(out, 0, I.n * I.W * sizeof(uint64_t));
+    return;
+  }
+  const uint64_t chunk = stage_words / I.W;
+  if (chunk == 0)
+    fail(HB_E_INVALID_ARG, "a counter of " + std::to_string(I.W * 8) +
+                               " bytes does not fit the staging buffer");
+  const uint32_t *src = I.cnt[which ? 1 - I.cur : I.cur].p;
+  for (uint64_t b = 0; b < I.n; b += chunk) {
+    const uint64_t c = std::min(chunk, I.n - b);
+    HB_HIP(launch_to_reference_layout(src, b, c, I.r, I.m, stage, I.stream));
+    HB_HIP(hipMemcpyAsync(out + b * I.W, stage, c * I.W * 8, hipMemcpyDeviceToHost, I.stream));
+    HB_HIP(hipStreamSynchronize(I.stream));
+  }
+}
+
 hb_status hb_get_counters(hb_instance *hb, int which, uint64_t *out, size_t words) {
   return guard([&] {
     Instance &I = *checked(hb);
-    if (!out) fail(HB_E_INVALID_ARG, "null output");
-    if (!I.inited) fail(HB_E_NOT_RUN, "counters are not initialised; call hb_init or hb_run");
-    if (words < I.n * I.W) fail(HB_E_INVALID_ARG, "output buffer too small");
-    if (which && I.iteration == 0) {  // the other buffer before the first iteration
-      std::memset(out, 0, I.n * I.W * sizeof(uint64_t));
-      return;
-    }
     I.use_device();
-    const uint32_t *src = I.cnt[which ? 1 - I.cur : I.cur].p;
-    const uint64_t chunk = std::max<uint64_t>(1, (uint64_t(64) << 20) / (I.W * 8));
+    const uint64_t stage_words = std::min<uint64_t>(
+        std::max<uint64_t>(I.n, 1) * I.W, std::max<uint64_t>((uint64_t(64) << 20) / 8, I.W));
     DevBuf<uint64_t> tmp;
-    tmp.alloc(std::min<uint64_t>(I.n, chunk) * I.W);
-    for (uint64_t b = 0; b < I.n; b += chunk) {
-      const uint64_t c = std::min(chunk, I.n - b);
-      HB_HIP(launch_to_reference_layout(src, b, c, I.r, I.m, tmp.p, I.stream));
-      HB_HIP(hipMemcpyAsync(out + b * I.W, tmp.p, c * I.W * 8, hipMemcpyDeviceToHost, I.stream));
-      HB_HIP(hipStreamSynchronize(I.stream));
-    }
+    if (out && I.inited && words >= I.n * I.W && !(which && I.iteration == 0))
+      tmp.alloc(stage_words);
+    copy_counters(I, which, out, words, tmp.p, stage_words);
+  });
+}
+
+hb_status hb_get_counters_noalloc(hb_instance *hb, int which, uint64_t *out, size_t words) {
+  return guard([&] {
+    Instance &I = *checked(hb);
+    I.use_device();
+    copy_counters(I, which, out, words, scratch_address(I.device), kNoAllocScratchBytes / 8);
   });
 }
 

@@ -122,6 +122,7 @@ _SIGNATURES = [
     ("hb_reachable_nodes_from", _I, [_P, _U64, _P]),
     ("hb_get_params", _I, [_P, _P]),
     ("hb_get_counters", _I, [_P, _I, _P, _SZ]),
+    ("hb_get_counters_noalloc", _I, [_P, _I, _P, _SZ]),
     ("hb_get_modified", _I, [_P, _P, _SZ]),
     ("hb_num_iterations", _I, [_P, _P]),
     ("hb_get_iteration_stats", _I, [_P, _U64, _P]),
@@ -512,6 +513,14 @@ class HyperBall:
         W = self.params()["words_per_counter"]
         out = np.zeros((self._n, W), dtype=np.uint64)
         _check(lib().hb_get_counters(self._h, which, _ptr(out), out.size))
+        return out
+
+    def counters_noalloc(self, which: int = 0) -> np.ndarray:
+        """counters() through hb_get_counters_noalloc, which stages through the
+        library's per-GPU array instead of allocating device memory."""
+        W = self.params()["words_per_counter"]
+        out = np.zeros((self._n, W), dtype=np.uint64)
+        _check(lib().hb_get_counters_noalloc(self._h, which, _ptr(out), out.size))
         return out
 
     def modified_bits(self) -> np.ndarray:
