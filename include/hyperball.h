@@ -163,7 +163,7 @@ typedef struct hb_instance hb_instance;
  * .build(pl)  (hyperball_impl.rs:91-133, :321-403).  `transpose` may be NULL
  * (no systolic iterations).  Errors: HB_E_INVALID_ARG for log2m < 4, counters
  * that are not whole 64-bit words, wrong weight/transposed sizes, or shapes
- * this library does not hold (log2m above 12, register width other than 5 or
+ * this library does not hold (log2m above 16, register width other than 5 or
  * 6). */
 hb_status hb_create(const hb_csr *graph, const hb_csr *transpose, const hb_config *cfg,
                     hb_instance **out);

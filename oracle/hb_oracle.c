@@ -95,7 +95,7 @@ int or_hll_init(or_hll *h, uint32_t log2m, uint32_t reg_bits, uint64_t num_eleme
   uint64_t m = 1ull << log2m;
   uint64_t bits = m * r;
   if (bits % 64 != 0) return -2;
-  if (bits / 64 > 512) return -3;
+  if (bits / 64 > OR_MAX_WORDS) return -3;
   h->log2m = log2m;
   h->m = (uint32_t)m;
   h->r = r;
@@ -286,7 +286,7 @@ static void subtract_words(uint64_t *x, const uint64_t *y, uint32_t n) {
  *   x ^= (x ^ y) & mask */
 void or_merge_broadword(const or_hll *h, uint64_t *x, const uint64_t *y) {
   uint32_t W = h->W, r1 = h->r - 1;
-  uint64_t acc[512], mask[512];
+  uint64_t acc[OR_MAX_WORDS], mask[OR_MAX_WORDS];
   for (uint32_t i = 0; i < W; i++) {
     acc[i] = y[i] | h->msb[i];
     mask[i] = x[i] & ~h->msb[i];
@@ -519,7 +519,7 @@ static void parallel_task(or_hb *hb, uint64_t *node_cursor, uint64_t *arc_cursor
   const int do_centrality = hb->sd || hb->hc || hb->ndisc > 0;
   uint64_t visited = 0, modified = 0;
   double ks = 0.0, kc = 0.0; /* Kahan sum of the neighbourhood-function delta */
-  uint64_t next_counter[512];
+  uint64_t next_counter[OR_MAX_WORDS];
   uint64_t *pushes = NULL;
   size_t np = 0, npc = 0;
 #define KAHAN_ADD(x)                    \

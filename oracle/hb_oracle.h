@@ -31,6 +31,10 @@ extern "C" {
 
 enum { OR_HASH_SIPHASH13 = 0, OR_HASH_JENKINS = 1 };
 
+/* Largest counter the oracle holds, in 64-bit words: 2^16 registers of 6
+ * bits. */
+#define OR_MAX_WORDS 6144
+
 typedef struct {
   uint32_t log2m;     /* base-2 logarithm of the number of registers */
   uint32_t m;         /* registers per counter */
@@ -40,8 +44,8 @@ typedef struct {
   double alpha_m_m;   /* alpha * m^2 */
   int hash;           /* OR_HASH_* */
   uint64_t seed;      /* Jenkins seed */
-  uint64_t msb[512];  /* H_r: top bit of every register (W words) */
-  uint64_t lsb[512];  /* L_r: bottom bit of every register (W words) */
+  uint64_t msb[OR_MAX_WORDS];  /* H_r: top bit of every register (W words) */
+  uint64_t lsb[OR_MAX_WORDS];  /* L_r: bottom bit of every register (W words) */
 } or_hll;
 
 /* HyperLogLogBuilder statics, hyper_log_log_logic.rs:275-303 */
@@ -52,7 +56,7 @@ uint32_t or_log2_num_registers(double rsd);
 /* HyperLogLogBuilder::build, hyper_log_log_logic.rs:374-428.  reg_bits = 0
  * uses the reference's formula.  Returns 0, or -1 for log2m < 4, -2 when the
  * counter is not a whole number of 64-bit words, -3 for sizes this oracle does
- * not hold (W > 512 or r outside [4, 8]). */
+ * not hold (W > OR_MAX_WORDS or r outside [4, 8]). */
 int or_hll_init(or_hll *h, uint32_t log2m, uint32_t reg_bits, uint64_t num_elements,
                 int hash, uint64_t seed);
 

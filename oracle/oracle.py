@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
 
 HASH_SIPHASH13, HASH_JENKINS = 0, 1
+OR_MAX_WORDS = 6144  # hb_oracle.h
 
 
 def build() -> str:
@@ -28,7 +29,7 @@ class _Hll(ctypes.Structure):
     _fields_ = [("log2m", ctypes.c_uint32), ("m", ctypes.c_uint32), ("r", ctypes.c_uint32),
                 ("W", ctypes.c_uint32), ("sentinel", ctypes.c_uint64), ("alpha_m_m", ctypes.c_double),
                 ("hash", ctypes.c_int), ("seed", ctypes.c_uint64),
-                ("msb", ctypes.c_uint64 * 512), ("lsb", ctypes.c_uint64 * 512)]
+                ("msb", ctypes.c_uint64 * OR_MAX_WORDS), ("lsb", ctypes.c_uint64 * OR_MAX_WORDS)]
 
 
 _P, _U64, _U32, _D, _I = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double, ctypes.c_int

@@ -63,9 +63,10 @@ int main(int argc, char **argv) {
   int bad = 0, shapes = 0;
   std::mt19937_64 rng(12345);
   for (uint32_t r = 5; r <= 6; r++) {
-    for (uint32_t log2m = 5; log2m <= 12; log2m++) {
+    for (uint32_t log2m = 5; log2m <= 16; log2m++) {
       int lanes = 0, spl = 0;
-      if (!hb::union_shape(int(r), int(log2m), &lanes, &spl)) continue;
+      int nseg = 0;
+      if (!hb::union_shape(int(r), int(log2m), &lanes, &spl, &nseg)) continue;
       or_hll h;
       if (or_hll_init(&h, log2m, r, 0, OR_HASH_JENKINS, 0) != 0) continue;
       shapes++;

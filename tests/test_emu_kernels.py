@@ -49,6 +49,7 @@ def test_emulated_batch_matches_single_instances(emu_exes, graph, heavy, tune):
     (5, 6, 300, 250, 7, 64, 1),     # one slab, G = 1 (6 dwords, stride 8)
     (8, 5, 300, 250, 9, 64, 0),     # 40 dwords, stride 48
     (7, 5, 300, 250, 10, 0, 1),     # 20 dwords, stride 32
+    (13, 5, 200, 150, 6, 0, 1),     # two 64-lane segments per counter, delta gathering forced
 ])
 def test_emulated_engine_matches_oracle(emu_exes, log2m, r, n, dmax, seed, heavy, delta):
     """Every iteration (standard, systolic, local, pre-local) of the emulated
@@ -71,6 +72,6 @@ def test_emulated_count_high_registers_match_oracle(emu_exes):
     res = subprocess.run([emu_exes["oracle_count_check"], "36"], capture_output=True, text=True,
                          timeout=900)
     assert res.returncode == 0 and res.stdout.strip().endswith("OK"), res.stdout[-3000:] + res.stderr[-6000:]
-    assert res.stdout.count("identical") == 15
+    assert res.stdout.count("identical") == 23
     assert "registers up to 31: identical" in res.stdout
     assert "registers up to 63: identical" in res.stdout

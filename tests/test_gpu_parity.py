@@ -71,7 +71,8 @@ def test_tiny_graphs_bit_exact(hbm, orc, name, transpose):
 
 
 @pytest.mark.parametrize("log2m,reg_bits", [(5, 6), (6, 5), (7, 5), (8, 5), (9, 6), (10, 5), (10, 6),
-                                            (11, 5), (12, 5), (12, 6)])
+                                            (11, 5), (12, 5), (12, 6), (13, 5), (14, 6), (15, 5),
+                                            (16, 5), (16, 6)])
 def test_uniform_all_shapes_bit_exact(hbm, orc, log2m, reg_bits):
     g = hbm.Graph.uniform(3000, 3, seed=11)
     _run_lockstep(hbm, orc, g, g.transpose(), log2m, reg_bits, jenkins=7)
@@ -88,7 +89,8 @@ def test_web_systolic_local_bit_exact(hbm, orc, hash_):
     assert stats[first]["checked"] < g.num_nodes, "the first systolic iteration checked every node"
 
 
-@pytest.mark.parametrize("log2m,reg_bits", [(8, 5), (6, 6), (10, 6), (12, 6), (11, 5)])
+@pytest.mark.parametrize("log2m,reg_bits", [(8, 5), (6, 6), (10, 6), (12, 6), (11, 5), (13, 6),
+                                            (14, 5)])
 def test_engine_outputs_match_reference_engine(hbm, orc, log2m, reg_bits):
     g = hbm.Graph.web(30000, dmin=1, dmax=300, window=200, seed=9)
     gt = g.transpose()

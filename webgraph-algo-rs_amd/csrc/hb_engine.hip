@@ -89,6 +89,9 @@ struct Instance {
 
   uint64_t n = 0, a = 0;
   uint32_t log2m = 0, m = 0, r = 0, W = 0, cdw = 0;
+  uint32_t lanes = 1;  // lanes per node in the union kernel
+  uint32_t nseg = 1;   // counter segments: above 4096 registers a group works segment by
+                       // segment, and no row takes the heavy-row phase
   uint32_t cstride = 0;  // dwords between counters: counter_stride(cdw)
   uint64_t sentinel = 0;
   double amm = 0.0;
@@ -369,8 +372,6 @@ struct Instance {
     ua.chg_next = chg[1 - mcur].p;
     ua.chg_shift = chg_shift;
 
-    int lanes = 1, spl = 1;
-    union_shape(int(r), int(log2m), &lanes, &spl);
     const uint64_t npb = kBlock / lanes;
     p.grid = max_grid;
     if (!p.list_mode)
@@ -781,11 +782,11 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
       fail(HB_E_INVALID_ARG,
            "W should allow counter backends to be aligned. Use " + min_alignment(m * r) +
                " or smaller unsigned integer types");
-    int lanes, spl;
-    if (!union_shape(int(r), int(cfg->log2m), &lanes, &spl))
+    int lanes, spl, nseg;
+    if (!union_shape(int(r), int(cfg->log2m), &lanes, &spl, &nseg))
       fail(HB_E_INVALID_ARG, "no kernel for register width " + std::to_string(r) + " and log2m " +
                                  std::to_string(cfg->log2m) +
-                                 " (this library holds widths 5 and 6, log2m 5..12)");
+                                 " (this library holds widths 5 and 6, log2m 5..16)");
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -803,6 +804,8 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     auto inst = std::make_unique<Instance>();
     Instance &I = *inst;
     I.device = cfg->device;
+    I.lanes = uint32_t(lanes);
+    I.nseg = uint32_t(nseg);
     I.use_device();
     I.cus = uint32_t(prop.multiProcessorCount);
     if (shared_stream) {
@@ -895,7 +898,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
       std::vector<std::pair<uint64_t, uint32_t>> heavy;
       for (uint64_t v = 0; v < n; v++) {
         const uint64_t d = graph->offsets[v + 1] - graph->offsets[v];
-        if (d > Instance::kHeavyFloor) heavy.push_back({d, uint32_t(v)});
+        if (nseg == 1 && d > Instance::kHeavyFloor) heavy.push_back({d, uint32_t(v)});
       }
       std::sort(heavy.begin(), heavy.end(),
                 [](const auto &x, const auto &y) { return x.first > y.first || (x.first == y.first && x.second < y.second); });

@@ -132,12 +132,12 @@ __device__ __forceinline__ SlabSum5 slab_sum5(const uint32_t *p) {
 }
 
 struct SlabSum6 {
-  uint64_t hi;  // sum of 2^(11 - v) over registers with v < 12
-  uint64_t lo;  // sum of 2^(63 - v) over registers with v >= 12
+  uint64_t hi;  // sum of 2^(15 - v) over registers with v < 16
+  uint64_t lo;  // sum of 2^(63 - v) over registers with v >= 16
   uint32_t zeros;
 };
 
-// S = hi * 2^52 + lo.  For up to 2^12 registers hi < 2^24 and lo <= 2^63,
+// S = hi * 2^48 + lo.  For up to 2^16 registers hi < 2^32 and lo <= 2^63,
 // so neither part overflows 64 bits.
 __device__ __forceinline__ SlabSum6 slab_sum6(const uint32_t *p) {
   SlabSum6 out;
@@ -150,7 +150,7 @@ __device__ __forceinline__ SlabSum6 slab_sum6(const uint32_t *p) {
     const uint32_t qh = q45[h >> 2] & ((h & 2) ? p[3] : n[3]) & ((h & 1) ? p[2] : n[2]);
     const uint32_t a = qh & n[1], b = qh & p[1];
     const uint32_t t = 8 * __popc(a) - 4 * __popc(a & p[0]) + 2 * __popc(b) - __popc(b & p[0]);
-    if (h < 3) hi += uint64_t(t) << (8 - 4 * h);
+    if (h < 4) hi += uint64_t(t) << (12 - 4 * h);
     else lo += uint64_t(t) << (60 - 4 * h);
   }
   out.hi = hi;
@@ -158,44 +158,61 @@ __device__ __forceinline__ SlabSum6 slab_sum6(const uint32_t *p) {
   return out;
 }
 
-// count() of the counter whose slabs are spread over the G lanes of a group;
-// every lane returns the result.  hyper_log_log_logic.rs:169-186.
-template <int R, int G, int SPL>
-__device__ __forceinline__ double group_count(const uint32_t *x, double alpha_m_m, uint32_t m,
-                                              const double *lc_table) {
-  double hm;
-  uint32_t zeros = 0;
-  if constexpr (R == 5) {
-    int64_t s = 0;
+// Partial count() sums of one lane's slabs (SPL slabs of R planes at x),
+// added into (s, hi, lo, zeros): s for 5-bit registers, (hi, lo) for 6-bit
+// ones, as slab_sum5 / slab_sum6 define them.
+template <int R, int SPL>
+__device__ __forceinline__ void lane_sums(const uint32_t *x, int64_t &s, uint64_t &hi,
+                                          uint64_t &lo, uint32_t &zeros) {
 #pragma unroll
-    for (int k = 0; k < SPL; k++) {
-      SlabSum5 t = slab_sum5(x + k * R);
+  for (int k = 0; k < SPL; k++) {
+    if constexpr (R == 5) {
+      const SlabSum5 t = slab_sum5(x + k * R);
       s += t.s;
       zeros += t.zeros;
-    }
-    s = int64_t(group_sum_u64<G>(uint64_t(s)));
-    zeros = group_sum_u32<G>(zeros);
-    hm = double(s) * 0x1p-31;
-  } else {
-    uint64_t shi = 0, slo = 0;
-#pragma unroll
-    for (int k = 0; k < SPL; k++) {
-      SlabSum6 t = slab_sum6(x + k * R);
-      shi += t.hi;
-      slo += t.lo;
+    } else {
+      const SlabSum6 t = slab_sum6(x + k * R);
+      hi += t.hi;
+      lo += t.lo;
       zeros += t.zeros;
     }
+  }
+}
+
+// count() from the lane sums of a whole counter spread over the G lanes of a
+// group; every lane returns the result.  hyper_log_log_logic.rs:169-186.
+template <int R, int G>
+__device__ __forceinline__ double group_estimate(int64_t s, uint64_t shi, uint64_t slo,
+                                                 uint32_t zeros, double alpha_m_m, uint32_t m,
+                                                 const double *lc_table) {
+  double hm;
+  zeros = group_sum_u32<G>(zeros);
+  if constexpr (R == 5) {
+    s = int64_t(group_sum_u64<G>(uint64_t(s)));
+    hm = double(s) * 0x1p-31;
+  } else {
     shi = group_sum_u64<G>(shi);
     slo = group_sum_u64<G>(slo);
-    zeros = group_sum_u32<G>(zeros);
-    // S = shi * 2^52 + slo as a 128-bit integer (hi:lo words)
-    const uint64_t lo = slo + (shi << 52);
-    const uint64_t hi = (shi >> 12) + (lo < slo ? 1 : 0);
+    // S = shi * 2^48 + slo as a 128-bit integer (hi:lo words)
+    const uint64_t lo = slo + (shi << 48);
+    const uint64_t hi = (shi >> 16) + (lo < slo ? 1 : 0);
     hm = (double(hi) * 18446744073709551616.0 + double(lo)) * 0x1p-63;
   }
   double est = alpha_m_m / hm;
   if (zeros != 0 && est < 2.5 * double(m)) est = lc_table[zeros];
   return est;
+}
+
+// count() of the counter whose slabs are spread over the G lanes of a group;
+// every lane returns the result.
+template <int R, int G, int SPL>
+__device__ __forceinline__ double group_count(const uint32_t *x, double alpha_m_m, uint32_t m,
+                                              const double *lc_table) {
+  int64_t s = 0;
+  uint64_t hi = 0, lo = 0;
+  uint32_t zeros = 0;
+  lane_sums<R, SPL>(x, s, hi, lo, zeros);
+  return group_estimate<R, G>(s, hi, lo, zeros, alpha_m_m, m, lc_table);
 }
 
 template <int D>
@@ -370,30 +387,32 @@ __device__ __forceinline__ void seed_register(uint32_t hash, uint64_t seed, uint
 // counter or change mask is read.
 //
 // MODE: 0 whole counters, 1 delta gathering, 2 seeded counters.
-template <int R, int G, int SPL, int MODE>
+template <int R, int G, int SPL, int MODE, int NSEG = 1>
 __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int gl,
                                                  uint64_t first, uint64_t end, uint64_t stride,
                                                  uint32_t *prev, uint32_t *cur, bool &loaded,
                                                  uint32_t &nvisit, uint32_t &acc_gathered,
-                                                 uint32_t *ids) {
+                                                 uint32_t *ids, int seg = 0) {
   constexpr bool DELTA = MODE == 1, SEEDED = MODE == 2;
   constexpr int D = R * SPL;         // dwords per lane
-  constexpr int CD = G * D;          // dwords per counter
+  constexpr int CD = G * D;          // dwords per counter segment
   constexpr int K = ids_per_lane<G>();
 #ifdef HB_LOADS_IN_FLIGHT
   constexpr int U = HB_LOADS_IN_FLIGHT;
 #else
   constexpr int U = D <= 6 ? 8 : D <= 12 ? 4 : 2;  // successor loads in flight per lane
 #endif
-  constexpr uint32_t CS = counter_stride(CD);  // dwords between counters
-  const uint32_t *own = a.curr + uint64_t(v) * CS + gl * D;
-  const uint32_t *lane_base = a.curr + gl * D;
+  constexpr uint32_t CS = counter_stride(NSEG * CD);  // dwords between counters
+  const uint32_t *own = a.curr + uint64_t(v) * CS + seg * CD + gl * D;
+  const uint32_t *lane_base = a.curr + seg * CD + gl * D;
   uint32_t *masks = ids + slice_width<G>();
   const uint64_t below = (uint64_t(1) << gl) - 1;
+  // the counter's slab index of this lane's first slab
+  const uint32_t slab0 = uint32_t(seg) * G * SPL + uint32_t(gl) * SPL;
   // this lane's slabs in change-mask bits
   uint32_t lane_bit[SPL];
 #pragma unroll
-  for (int q = 0; q < SPL; q++) lane_bit[q] = 1u << ((uint32_t(gl) * SPL + q) >> a.chg_shift);
+  for (int q = 0; q < SPL; q++) lane_bit[q] = 1u << ((slab0 + q) >> a.chg_shift);
   // with every node checked most successors are modified: read the masks
   // together with the modified bits instead of after them
   const bool speculate = a.flags & F_CHECK_ALL;
@@ -456,7 +475,7 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
         const uint32_t reg = w & 0xffffu, val = w >> 16, slab = reg >> 5, bit = reg & 31;
 #pragma unroll
         for (int q = 0; q < SPL; q++) {
-          if (slab != uint32_t(gl) * SPL + q) continue;
+          if (slab != slab0 + q) continue;
           uint32_t cv = 0;
 #pragma unroll
           for (int b = 0; b < R; b++) cv |= ((cur[q * R + b] >> bit) & 1u) << b;
@@ -504,58 +523,38 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
   }
 }
 
-template <int R, int G, int SPL>
+template <int R, int G, int SPL, int NSEG = 1>
 __device__ __forceinline__ void merge_successors(const UnionArgs &a, uint32_t v, int gl,
                                                  uint64_t first, uint64_t end, uint64_t stride,
                                                  uint32_t *prev, uint32_t *cur, bool &loaded,
                                                  uint32_t &nvisit, uint32_t &acc_gathered,
-                                                 uint32_t *ids) {
+                                                 uint32_t *ids, int seg = 0) {
   if (a.flags & F_SEEDED)
-    merge_slices<R, G, SPL, 2>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                               acc_gathered, ids);
+    merge_slices<R, G, SPL, 2, NSEG>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                     acc_gathered, ids, seg);
   else if (a.flags & F_DELTA)
-    merge_slices<R, G, SPL, 1>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                               acc_gathered, ids);
+    merge_slices<R, G, SPL, 1, NSEG>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                     acc_gathered, ids, seg);
   else
-    merge_slices<R, G, SPL, 0>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                               acc_gathered, ids);
+    merge_slices<R, G, SPL, 0, NSEG>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                     acc_gathered, ids, seg);
 }
 
-// Everything after the successor loop (hyperball_impl.rs:1051-1144): changed
-// test, count(), neighbourhood function, centralities, modified bit,
-// predecessor marking and the store into the next state.
-template <int R, int G, int SPL>
-__device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int gl, bool check,
-                                            bool was_mod, bool loaded, const uint32_t *prev,
-                                            uint32_t *cur, uint32_t nvisit, uint64_t deg,
-                                            Acc &acc_out) {
-  constexpr int D = R * SPL, CD = G * D;
+// The per-node bookkeeping after the successor loop (hyperball_impl.rs:
+// 1051-1130), once the group knows whether the counter changed (cm), its
+// changed slabs (chg_bits, in change-mask bits, the same on every lane) and
+// its new count (post, when cm): count cache, neighbourhood function,
+// centralities, modified bit, change mask and predecessor marking.
+template <int G>
+__device__ __forceinline__ void node_update(const UnionArgs &a, uint32_t v, int gl, bool check,
+                                            bool cm, uint32_t chg_bits, double post,
+                                            uint32_t nvisit, uint64_t deg, Acc &acc_out) {
   const bool systolic = a.flags & F_SYSTOLIC;
-  bool cm = false;
-  uint32_t chg_bits = 0;  // this lane's changed slabs, in change-mask bits
-  if (loaded) {
-#pragma unroll
-    for (int q = 0; q < SPL; q++) {
-      uint32_t d = 0;
-#pragma unroll
-      for (int b = 0; b < R; b++) d |= cur[q * R + b] ^ prev[q * R + b];
-      if (d) chg_bits |= 1u << ((uint32_t(gl) * SPL + q) >> a.chg_shift);
-    }
-    cm = group_ballot<G>(chg_bits != 0) != 0;
-  }
-  if (cm) {
-    chg_bits = group_or_u32<G>(chg_bits);
-    if (gl == 0) {
-      a.chg_next[v] = chg_bits;
-      acc_out.changed += __popc(chg_bits);
-    }
-  }
-  double post = 0.0;
-  if (cm) post = group_count<R, G, SPL>(cur, a.alpha_m_m, a.m, a.lc_table);
-
   if (gl == 0) {
     double pre = 0.0;
     if (cm) {
+      a.chg_next[v] = chg_bits;
+      acc_out.changed += __popc(chg_bits);
       pre = a.cache[v];
       a.cache[v] = post;
     } else if (!systolic && check) {
@@ -589,6 +588,35 @@ __device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int 
     }
     if (gl == 0) acc_out.tscanned += te - tb;
   }
+}
+
+// Everything after the successor loop (hyperball_impl.rs:1051-1144) for a
+// counter held whole by the group: changed test, count(), node_update, and
+// the store into the next state.
+template <int R, int G, int SPL>
+__device__ __forceinline__ void finish_node(const UnionArgs &a, uint32_t v, int gl, bool check,
+                                            bool was_mod, bool loaded, const uint32_t *prev,
+                                            uint32_t *cur, uint32_t nvisit, uint64_t deg,
+                                            Acc &acc_out) {
+  constexpr int D = R * SPL, CD = G * D;
+  bool cm = false;
+  uint32_t chg_bits = 0;  // this lane's changed slabs, in change-mask bits
+  if (loaded) {
+#pragma unroll
+    for (int q = 0; q < SPL; q++) {
+      uint32_t d = 0;
+#pragma unroll
+      for (int b = 0; b < R; b++) d |= cur[q * R + b] ^ prev[q * R + b];
+      if (d) chg_bits |= 1u << ((uint32_t(gl) * SPL + q) >> a.chg_shift);
+    }
+    cm = group_ballot<G>(chg_bits != 0) != 0;
+  }
+  double post = 0.0;
+  if (cm) {
+    chg_bits = group_or_u32<G>(chg_bits);
+    post = group_count<R, G, SPL>(cur, a.alpha_m_m, a.m, a.lc_table);
+  }
+  node_update<G>(a, v, gl, check, cm, chg_bits, post, nvisit, deg, acc_out);
 
   if (cm || was_mod) {
     if (!loaded) load_part<D>(cur, a.curr + uint64_t(v) * counter_stride(CD) + gl * D);
@@ -653,7 +681,7 @@ __device__ __forceinline__ void stage_chunk(const UnionArgs &a, uint64_t chunk, 
   lane_sync<64>();
 }
 
-template <int R, int G, int SPL>
+template <int R, int G, int SPL, int NSEG>
 __device__ __forceinline__ void process_node(const UnionArgs &a, const NodeMeta &m, int gl,
                                              Acc &acc_out, uint32_t *ids) {
   constexpr int D = R * SPL;
@@ -663,14 +691,66 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, const NodeMeta 
   const uint64_t beg = m.beg, end = m.end;
 
   uint32_t prev[D], cur[D];
-  bool loaded = false;
   uint32_t nvisit = 0, gathered = 0;
-  if (check)
-    merge_successors<R, G, SPL>(a, v, gl, beg, end, slice_width<G>(), prev, cur, loaded, nvisit,
-                                gathered, ids);
-  acc_out.gathered += gathered;
-  finish_node<R, G, SPL>(a, v, gl, check, was_mod, loaded, prev, cur, nvisit,
-                         check ? end - beg : 0, acc_out);
+  if constexpr (NSEG == 1) {
+    bool loaded = false;
+    if (check)
+      merge_successors<R, G, SPL>(a, v, gl, beg, end, slice_width<G>(), prev, cur, loaded,
+                                  nvisit, gathered, ids);
+    acc_out.gathered += gathered;
+    finish_node<R, G, SPL>(a, v, gl, check, was_mod, loaded, prev, cur, nvisit,
+                           check ? end - beg : 0, acc_out);
+  } else {
+    // Counters of NSEG segments of G * D dwords: the row is merged segment by
+    // segment.  The modified successors found are the same in every segment,
+    // so a segment is loaded exactly when all are; count() sums accumulate
+    // over the segments.  A lane stores its part of a segment when the node
+    // was modified last iteration (the other buffer is stale) or when the
+    // part changed; an unchanged part of an unmodified node already holds the
+    // same registers in the other buffer.
+    constexpr int CD = G * D;
+    constexpr uint32_t CS = counter_stride(NSEG * CD);
+    int64_t s5 = 0;
+    uint64_t hi = 0, lo = 0;
+    uint32_t zeros = 0, chg_bits = 0;
+    bool loaded = false;
+#pragma unroll 1
+    for (int seg = 0; seg < NSEG; seg++) {
+      uint32_t nv = 0;
+      loaded = false;
+      if (check)
+        merge_successors<R, G, SPL, NSEG>(a, v, gl, beg, end, slice_width<G>(), prev, cur,
+                                          loaded, nv, gathered, ids, seg);
+      if (seg == 0) nvisit = nv;
+      bool part_changed = false;
+      if (loaded) {
+#pragma unroll
+        for (int q = 0; q < SPL; q++) {
+          uint32_t d = 0;
+#pragma unroll
+          for (int b = 0; b < R; b++) d |= cur[q * R + b] ^ prev[q * R + b];
+          if (d) {
+            chg_bits |= 1u << ((uint32_t(seg) * G * SPL + uint32_t(gl) * SPL + q) >> a.chg_shift);
+            part_changed = true;
+          }
+        }
+        lane_sums<R, SPL>(cur, s5, hi, lo, zeros);
+      }
+      if (was_mod || part_changed) {
+        const uint64_t at = uint64_t(v) * CS + seg * CD + gl * D;
+        if (!loaded) load_part<D>(cur, a.curr + at);
+        store_part<D>(a.next + at, cur);
+      }
+    }
+    acc_out.gathered += gathered;
+    const bool cm = loaded && group_ballot<G>(chg_bits != 0) != 0;
+    double post = 0.0;
+    if (cm) {
+      chg_bits = group_or_u32<G>(chg_bits);
+      post = group_estimate<R, G>(s5, hi, lo, zeros, a.alpha_m_m, a.m, a.lc_table);
+    }
+    node_update<G>(a, v, gl, check, cm, chg_bits, post, nvisit, check ? end - beg : 0, acc_out);
+  }
 }
 
 
@@ -680,7 +760,7 @@ __device__ __forceinline__ void process_node(const UnionArgs &a, const NodeMeta 
 // groups take consecutive id-and-mask slots.  The regions are per wavefront, not per
 // shape: in the batched kernel the wavefronts of one workgroup may run
 // different shapes at the same time.
-template <int R, int G, int SPL>
+template <int R, int G, int SPL, int NSEG = 1>
 __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, uint64_t count,
                                           uint32_t rounds, uint32_t *wave_ids, NodeMeta *meta,
                                           Acc &acc) {
@@ -691,7 +771,7 @@ __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, ui
 #pragma unroll 1
   for (uint32_t round = 0; round < rounds; round++) {
     const NodeMeta m = meta[round * NPW + g];
-    if (m.flags & kMetaRun) process_node<R, G, SPL>(a, m, gl, acc, ids);
+    if (m.flags & kMetaRun) process_node<R, G, SPL, NSEG>(a, m, gl, acc, ids);
   }
   lane_sync<64>();  // the next chunk's staging overwrites meta
 }
@@ -801,14 +881,16 @@ __device__ __forceinline__ void heavy_phase(const UnionArgs &a, UnionLds<R, G, S
 
 // The union step of one iteration: first the rows longer than heavy_degree,
 // a workgroup per row (heavy_phase), then chunks of the other nodes, a lane
-// group per node.  Shapes of 5 dwords per lane fit four wavefronts per SIMD
-// (at most 128 VGPRs); the hint keeps the register allocator there.
-template <int R, int G, int SPL>
+// group per node.  Counters of several segments (NSEG > 1, more than 4096
+// registers) have no heavy-row phase: the host leaves heavy_count at 0.
+// Shapes of 5 dwords per lane fit four wavefronts per SIMD (at most 128
+// VGPRs); the hint keeps the register allocator there.
+template <int R, int G, int SPL, int NSEG>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(R * SPL <= 5 ? 4 : 1)))
 void union_kernel(UnionArgs a) {
   __shared__ UnionLds<R, G, SPL> L;
   Acc acc;
-  if (a.heavy_count) heavy_phase<R, G, SPL>(a, L, acc);
+  if (NSEG == 1 && a.heavy_count) heavy_phase<R, G, SPL>(a, L, acc);
   const uint64_t count = (a.flags & F_LIST) ? uint64_t(*a.list_len) : a.count;
   if (blockIdx.x == 0 && threadIdx.x == 0) a.stats->list_len = count;
   const uint32_t rounds = __builtin_amdgcn_readfirstlane(
@@ -823,7 +905,7 @@ void union_kernel(UnionArgs a) {
   if (__lane_id() == 0) c = first_chunk(a.work, chunks);
   c = __shfl(c, 0);
   while (c != UINT64_MAX) {
-    run_chunk<R, G, SPL>(a, c, count, rounds, wave_ids, meta, acc);
+    run_chunk<R, G, SPL, NSEG>(a, c, count, rounds, wave_ids, meta, acc);
     if (__lane_id() == 0) c = grab_chunk(a.work, chunks);
     c = __shfl(c, 0);
   }
@@ -837,25 +919,30 @@ void union_kernel(UnionArgs a) {
 // harmonic sum above m - 1, so the raw estimate alpha_m_m / sum stays below
 // 0.7214 (m + 2) < 2.5 m: count() is the linear-counting value
 // lc[m - 1] for every node, as group_count returns it.  Padding dwords beyond
-// the counter are not written; nothing reads them.
-template <int R, int G, int SPL>
+// the counter are not written; nothing reads them.  Counters of NSEG
+// segments are written segment by segment.
+template <int R, int G, int SPL, int NSEG>
 __global__ __launch_bounds__(kBlock) void seed_count_kernel(SeedArgs a, uint32_t m,
                                                            const double *lc, double *out) {
-  constexpr int D = R * SPL, NPB = kBlock / G;
+  constexpr int D = R * SPL, CD = G * D, NPB = kBlock / G;
   const int g = threadIdx.x / G, gl = threadIdx.x % G;
   for (uint64_t batch = blockIdx.x; batch * NPB < a.n; batch += gridDim.x) {
     const uint64_t i = batch * NPB + g;
     if (i < a.n) {
       uint32_t reg = 0, val = 0;
       seed_register(a.hash, a.seed, a.log2m, a.sentinel, i, reg, val);
-      uint32_t x[D];
+#pragma unroll 1
+      for (int seg = 0; seg < NSEG; seg++) {
+        uint32_t x[D];
 #pragma unroll
-      for (int q = 0; q < SPL; q++) {
-        const bool mine = (reg >> 5) == uint32_t(gl) * SPL + q;
+        for (int q = 0; q < SPL; q++) {
+          const bool mine = (reg >> 5) == uint32_t(seg) * G * SPL + uint32_t(gl) * SPL + q;
 #pragma unroll
-        for (int b = 0; b < R; b++) x[q * R + b] = mine && ((val >> b) & 1u) ? 1u << (reg & 31) : 0u;
+          for (int b = 0; b < R; b++)
+            x[q * R + b] = mine && ((val >> b) & 1u) ? 1u << (reg & 31) : 0u;
+        }
+        store_part<D>(a.counters + i * a.cdw + seg * CD + gl * D, x);
       }
-      store_part<D>(a.counters + i * a.cdw + gl * D, x);
       if (gl == 0) {
         out[i] = lc[m - 1];
         a.chg[i] = 1u << ((reg >> 5) >> a.chg_shift);
@@ -864,7 +951,7 @@ __global__ __launch_bounds__(kBlock) void seed_count_kernel(SeedArgs a, uint32_t
   }
 }
 
-template <int R, int G, int SPL>
+template <int R, int G, int SPL, int NSEG>
 __global__ __launch_bounds__(kBlock) void count_all_kernel(const uint32_t *cnt, uint64_t n,
                                                           double alpha_m_m, uint32_t m,
                                                           const double *lc, double *out) {
@@ -873,9 +960,16 @@ __global__ __launch_bounds__(kBlock) void count_all_kernel(const uint32_t *cnt, 
   for (uint64_t batch = blockIdx.x; batch * NPB < n; batch += gridDim.x) {
     const uint64_t i = batch * NPB + g;
     if (i < n) {
-      uint32_t x[D];
-      load_part<D>(x, cnt + i * counter_stride(CD) + gl * D);
-      const double c = group_count<R, G, SPL>(x, alpha_m_m, m, lc);
+      int64_t s = 0;
+      uint64_t hi = 0, lo = 0;
+      uint32_t zeros = 0;
+#pragma unroll 1
+      for (int seg = 0; seg < NSEG; seg++) {
+        uint32_t x[D];
+        load_part<D>(x, cnt + i * counter_stride(NSEG * CD) + seg * CD + gl * D);
+        lane_sums<R, SPL>(x, s, hi, lo, zeros);
+      }
+      const double c = group_estimate<R, G>(s, hi, lo, zeros, alpha_m_m, m, lc);
       if (gl == 0) out[i] = c;
     }
   }
@@ -1355,44 +1449,51 @@ uint32_t resident_grid(K kernel, std::atomic<uint32_t> *cache) {
   return g;
 }
 
-template <int R, int G, int SPL>
+template <int R, int G, int SPL, int NSEG>
 hipError_t launch_union_t(const UnionArgs &a, uint32_t grid, hipStream_t s) {
   static std::atomic<uint32_t> cache[kMaxDevices];
-  const uint32_t cap = resident_grid(union_kernel<R, G, SPL>, cache);
+  const uint32_t cap = resident_grid(union_kernel<R, G, SPL, NSEG>, cache);
   if (cap && grid > cap) grid = cap;
-  union_kernel<R, G, SPL><<<grid, kBlock, 0, s>>>(a);
+  union_kernel<R, G, SPL, NSEG><<<grid, kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
 
-template <int R, int G, int SPL>
+template <int R, int G, int SPL, int NSEG>
 hipError_t launch_seed_count_t(const SeedArgs &a, uint32_t m, const double *lc, double *out,
                                hipStream_t s) {
-  seed_count_kernel<R, G, SPL><<<grid_for(a.n, kBlock / G, 65536), kBlock, 0, s>>>(a, m, lc, out);
+  seed_count_kernel<R, G, SPL, NSEG><<<grid_for(a.n, kBlock / G, 65536), kBlock, 0, s>>>(a, m, lc,
+                                                                                        out);
   return hipGetLastError();
 }
 
-template <int R, int G, int SPL>
+template <int R, int G, int SPL, int NSEG>
 hipError_t launch_count_t(const uint32_t *c, uint64_t n, double amm, uint32_t m, const double *lc,
                           double *out, hipStream_t s) {
-  count_all_kernel<R, G, SPL><<<grid_for(n, kBlock / G, 65536), kBlock, 0, s>>>(c, n, amm, m, lc, out);
+  count_all_kernel<R, G, SPL, NSEG><<<grid_for(n, kBlock / G, 65536), kBlock, 0, s>>>(c, n, amm, m,
+                                                                                     lc, out);
   return hipGetLastError();
 }
 
 }  // namespace
 
-#define HB_SHAPES(X)                                                                           \
-  X(5, 5, 1, 1) X(5, 6, HB_G_6, HB_SPL_6) X(5, 7, 4, 1) X(5, 8, HB_G_8, HB_SPL_8)              \
-  X(5, 9, 8, 2) X(5, 10, HB_G_10, HB_SPL_10) X(5, 11, 32, 2) X(5, 12, 64, 2) X(6, 5, 1, 1)        \
-  X(6, 6, HB_G_6, HB_SPL_6) X(6, 7, 4, 1) X(6, 8, HB_G_8, HB_SPL_8) X(6, 9, 8, 2)               \
-  X(6, 10, HB_G_10, HB_SPL_10)                                                                 \
-  X(6, 11, 32, 2) X(6, 12, 64, 2)
+// (r, log2m, lanes per node, slabs per lane, segments per counter).  Counters
+// above 4096 registers are worked on one 64-lane segment of 128 slabs at a
+// time.
+#define HB_SHAPES(X)                                                                             \
+  X(5, 6, HB_G_6, HB_SPL_6, 1) X(5, 7, 4, 1, 1) X(5, 8, HB_G_8, HB_SPL_8, 1) X(5, 9, 8, 2, 1)     \
+  X(5, 10, HB_G_10, HB_SPL_10, 1) X(5, 11, 32, 2, 1) X(5, 12, 64, 2, 1) X(5, 13, 64, 2, 2)       \
+  X(5, 14, 64, 2, 4) X(5, 15, 64, 2, 8) X(5, 16, 64, 2, 16) X(6, 5, 1, 1, 1)                    \
+  X(6, 6, HB_G_6, HB_SPL_6, 1) X(6, 7, 4, 1, 1) X(6, 8, HB_G_8, HB_SPL_8, 1) X(6, 9, 8, 2, 1)     \
+  X(6, 10, HB_G_10, HB_SPL_10, 1) X(6, 11, 32, 2, 1) X(6, 12, 64, 2, 1) X(6, 13, 64, 2, 2)       \
+  X(6, 14, 64, 2, 4) X(6, 15, 64, 2, 8) X(6, 16, 64, 2, 16)
 
-bool union_shape(int r, int log2m, int *lanes, int *spl) {
-#define X(RR, LM, GG, SS)  \
-  if (r == RR && log2m == LM) { \
-    *lanes = GG;                \
-    *spl = SS;                  \
-    return true;                \
+bool union_shape(int r, int log2m, int *lanes, int *spl, int *nseg) {
+#define X(RR, LM, GG, SS, NS) \
+  if (r == RR && log2m == LM) {   \
+    *lanes = GG;                  \
+    *spl = SS;                    \
+    *nseg = NS;                   \
+    return true;                  \
   }
   HB_SHAPES(X)
 #undef X
@@ -1400,8 +1501,8 @@ bool union_shape(int r, int log2m, int *lanes, int *spl) {
 }
 
 hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s) {
-#define X(RR, LM, GG, SS) \
-  if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS>(a, grid, s);
+#define X(RR, LM, GG, SS, NS) \
+  if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS, NS>(a, grid, s);
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
@@ -1468,8 +1569,8 @@ hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
 hipError_t launch_count_all(int r, int log2m, const uint32_t *c, uint64_t n, double amm, uint32_t m,
                             const double *lc, double *out, hipStream_t s) {
   if (n == 0) return hipSuccess;
-#define X(RR, LM, GG, SS) \
-  if (r == RR && log2m == LM) return launch_count_t<RR, GG, SS>(c, n, amm, m, lc, out, s);
+#define X(RR, LM, GG, SS, NS) \
+  if (r == RR && log2m == LM) return launch_count_t<RR, GG, SS, NS>(c, n, amm, m, lc, out, s);
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
@@ -1479,8 +1580,9 @@ hipError_t launch_seed_count(const SeedArgs &a, uint32_t m, const double *lc, do
                              hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   if (a.weights) return hipErrorInvalidValue;
-#define X(RR, LM, GG, SS) \
-  if (int(a.r) == RR && int(a.log2m) == LM) return launch_seed_count_t<RR, GG, SS>(a, m, lc, out, s);
+#define X(RR, LM, GG, SS, NS)                                      \
+  if (int(a.r) == RR && int(a.log2m) == LM)                          \
+    return launch_seed_count_t<RR, GG, SS, NS>(a, m, lc, out, s);
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;

@@ -185,9 +185,11 @@ struct BatchLaunch {
 // per shape the heavy-row and union kernels.
 hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s);
 
-// Chooses the lane group of a (r, log2m) shape: lanes per node and slabs per
-// lane.  Returns false for shapes without a kernel.
-bool union_shape(int r, int log2m, int *lanes, int *slabs_per_lane);
+// Chooses the lane group of a (r, log2m) shape: lanes per node, slabs per
+// lane and segments per counter (a group works on one segment of lanes *
+// slabs_per_lane slabs at a time).  Returns false for shapes without a
+// kernel.
+bool union_shape(int r, int log2m, int *lanes, int *slabs_per_lane, int *segments);
 
 // Launches union_kernel (grid workgroups, capped at the resident ones): the
 // heavy rows of a.heavy_list first, a workgroup each, then the chunks.
