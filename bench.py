@@ -21,7 +21,9 @@ Beside the headline line the same JSON object carries:
                 per-node accumulators against f64 (max / p99 relative error);
   large         n = 10^8, about 2.1e9 arcs, log2m 6, 5-bit registers, FP32
                 Kahan accumulators, with the union step's roofline;
-  uniform_1e5   the uniform n = 10^5 graph, GPU against the oracle engine.
+  uniform_1e5   the uniform n = 10^5 graph, GPU against the oracle engine;
+  host_counters counter arrays in HBM, in host memory read in place, and in
+                host memory streamed through device memory in tiles.
 """
 import argparse
 import importlib.util
@@ -335,19 +337,26 @@ def bench_precision(hbm, args):
 
 
 def bench_host_counters(hbm, args):
-    """Counter arrays in HBM against pinned host memory (COUNTERS_HOST, the
-    kernels read and write them over the host link): the web graph with
-    n = args.host_nodes, log2m 8, 6-bit registers, systolic frontier, sum of
-    distances and harmonic; one run to stabilisation each, after a warmup
-    run, results compared."""
+    """Counter arrays in HBM against pinned host memory, read and written by
+    the kernels over the host link (COUNTERS_HOST), and against host memory
+    streamed through device memory in tiles (COUNTERS_STREAM: a tile of n/2,
+    then of n/8 nodes, so that 2 or 8 tiles' counters never share the device):
+    the web graph with n = args.host_nodes, log2m 8, 6-bit registers,
+    systolic frontier, sum of distances and harmonic; one run to
+    stabilisation each, after a warmup run, results compared."""
     g = hbm.Graph.web(args.host_nodes, 2.1, 3, 10000, 1000, 0.7, 2)
     gt = g.transpose()
     out = {"workload": f"web graph n={g.num_nodes} arcs={g.num_arcs}, log2m 8, r 6, jenkins(42)",
            "unit": "arc-registers/s"}
     results = []
-    for name, where in (("device", hbm.COUNTERS_DEVICE), ("host", hbm.COUNTERS_HOST)):
+    half = (g.num_nodes + 1) // 2
+    eighth = (g.num_nodes + 7) // 8
+    for name, where, tile in (("device", hbm.COUNTERS_DEVICE, 0), ("host", hbm.COUNTERS_HOST, 0),
+                              ("stream_2_tiles", hbm.COUNTERS_STREAM, half),
+                              ("stream_8_tiles", hbm.COUNTERS_STREAM, eighth)):
         hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).register_bits(6).jenkins(42)
-              .sum_of_distances(True).sum_of_inverse_distances(True).counter_memory(where).build())
+              .sum_of_distances(True).sum_of_inverse_distances(True).counter_memory(where, tile)
+              .build())
         hb.run_until_done()
         t0 = time.perf_counter()
         hb.run_until_done()
@@ -355,11 +364,16 @@ def bench_host_counters(hbm, args):
         st = hb.iteration_stats()
         work = sum(s["visited_arcs"] for s in st) * 256
         out[name] = {"ms_per_run": secs * 1e3, "value": work / secs, "iterations": len(st)}
+        if tile:
+            out[name]["tile_nodes"] = tile
         results.append((hb.counters(0), hb.sum_of_distances(), hb.neighborhood_function()))
         hb.close()
         print(f"# {name} counters: {secs * 1e3:.1f} ms per run", file=sys.stderr, flush=True)
-    out["identical_results"] = all(bool(np.array_equal(a, b)) for a, b in zip(*results))
+    out["identical_results"] = all(bool(np.array_equal(a, b)) for r in results[1:]
+                                   for a, b in zip(results[0], r))
     out["host_over_device"] = out["host"]["ms_per_run"] / out["device"]["ms_per_run"]
+    for k in ("stream_2_tiles", "stream_8_tiles"):
+        out[k]["speedup_over_host"] = out["host"]["ms_per_run"] / out[k]["ms_per_run"]
     return out
 
 
@@ -464,6 +478,8 @@ def main():
     ap.add_argument("--host-counters", action="store_true",
                     help="only the counters-in-host-memory comparison (n = --host-nodes)")
     ap.add_argument("--host-nodes", type=int, default=1_000_000)
+    ap.add_argument("--no-host-counters", action="store_true",
+                    help="skip the host / streamed counter-memory leg")
     ap.add_argument("--stats", action="store_true", help="print per-iteration statistics to stderr")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size "
@@ -608,6 +624,8 @@ def main():
         result["large"] = bench_large(hbm, args, compare_transposes=False)
     if not args.no_small:
         result["uniform_1e5"] = small_uniform(hbm, orc, args)
+    if not args.no_host_counters:
+        result["host_counters"] = bench_host_counters(hbm, args)
     print(json.dumps(result), flush=True)
 
 

@@ -133,6 +133,13 @@ uint32_t hb_log2_num_registers(double rsd);  /* HyperLogLog::log_2_num_of_regist
                                 host link: instances whose counters exceed HBM; every other
                                 buffer stays in HBM.  No file-backed mode: the memory must be
                                 allocated pinned. */
+#define HB_COUNTERS_STREAM 2 /* pinned host memory, streamed through device memory: each
+                                iteration walks the nodes in tiles of stream_tile_nodes nodes;
+                                a tile's own counters and then every block (a quarter of a
+                                tile) of the current array that holds modified nodes are
+                                copied to the device, merged there, and the tile's next
+                                counters copied back.  Device memory holds about 4.5 tiles of
+                                counters; every other buffer stays in HBM. */
 
 typedef struct {
   uint32_t log2m;        /* log2 of registers per counter (with_hyper_log_log `log2m`) */
@@ -149,6 +156,8 @@ typedef struct {
   int device;            /* HIP device ordinal */
   int device_transpose;  /* with no transpose given: build it on the device (systolic mode) */
   uint32_t counter_memory; /* HB_COUNTERS_* */
+  uint64_t stream_tile_nodes; /* HB_COUNTERS_STREAM: nodes per tile, rounded up to a multiple of
+                                 128 (0: the tile whose counters take 1 GiB) */
 } hb_config;
 
 /* Fills the defaults of HyperBallBuilder::with_hyper_log_log (:91-133):
@@ -244,6 +253,8 @@ typedef struct {
   uint64_t gathered_slabs;/* successor slabs (32 registers each) actually read */
   uint32_t delta_gathering; /* 1 when only the slabs that changed in the previous iteration
                                were gathered (chosen per iteration) */
+  uint32_t streamed;      /* HB_COUNTERS_STREAM: 1 when the iteration streamed the counters
+                             through device memory, 0 when its kernels read them in place */
 } hb_iteration_stats;
 hb_status hb_num_iterations(hb_instance *hb, uint64_t *out);
 hb_status hb_get_iteration_stats(hb_instance *hb, uint64_t iteration, hb_iteration_stats *out);
@@ -269,6 +280,16 @@ hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree);
  * iteration, when counters have at least 1024 registers and the expected
  * share of changed slabs is below 0.7; 1: always; 2: never. */
 hb_status hb_set_delta_gathering(hb_instance *hb, int mode);
+/* Streamed counters (HB_COUNTERS_STREAM): per iteration, either stream the
+ * counter arrays through device memory tile by tile, or let the union kernel
+ * read and write them in place over the host link, as HB_COUNTERS_HOST does.
+ * mode 0 (default): in place for the first iteration without weights (it
+ * reads no counter: the seeded registers are computed from hashes) and for
+ * iterations expected to visit fewer arcs than max(1, tiles - 2) * n
+ * (estimated as the last iteration's modified nodes times the mean
+ * out-degree), streamed otherwise; 1: always streamed; 2: always in place.
+ * Results do not depend on the mode. */
+hb_status hb_set_streaming(hb_instance *hb, int mode);
 /* Autotuner of that split (in the role of the reference's arc_granularity
  * tuning knob, hyperball_impl.rs:271-274, which sizes CPU work units): times
  * the first, dense iteration under each threshold 64 * 2^k that changes the

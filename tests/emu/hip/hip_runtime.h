@@ -88,6 +88,7 @@ enum hipStreamCaptureMode {
 };
 #define hipStreamNonBlocking 1u
 #define hipHostMallocDefault 0u
+#define hipEventDisableTiming 2u
 
 struct EmuStream;
 struct EmuEvent;
@@ -133,6 +134,10 @@ hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
 hipError_t hipStreamCreate(hipStream_t *s);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipEventCreate(hipEvent_t *e);
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
+// streams run each operation when it is issued, so an event has always
+// completed by the time a stream could wait for it
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);

@@ -3,7 +3,8 @@
 // every iteration the current registers (reference layout), the modified
 // bits and the iteration mode must be identical; at the end, the sums of
 // distances.  Usage:
-//   oracle_check LOG2M REG_BITS N DMAX SEED HEAVY [DELTA_MODE]
+//   oracle_check LOG2M REG_BITS N DMAX SEED HEAVY [DELTA_MODE [COUNTER_MEMORY TILE_NODES
+//                [STREAMING_MODE]]]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -33,6 +34,9 @@ int main(int argc, char **argv) {
   const uint64_t seed = std::strtoull(argv[5], nullptr, 10);
   const uint32_t heavy = uint32_t(std::atoi(argv[6]));
   const int delta_mode = argc > 7 ? std::atoi(argv[7]) : 0;
+  const uint32_t counter_memory = argc > 8 ? uint32_t(std::atoi(argv[8])) : HB_COUNTERS_DEVICE;
+  const uint64_t tile_nodes = argc > 9 ? std::strtoull(argv[9], nullptr, 10) : 0;
+  const int streaming = argc > 10 ? std::atoi(argv[10]) : 0;
 
   hb_graph gg, gt;
   check(hb_graph_generate_web(n, 2.1, 1, dmax, 100, 0.7, seed, &gg), "generate");
@@ -46,10 +50,13 @@ int main(int argc, char **argv) {
   cfg.hash = HB_HASH_JENKINS;
   cfg.seed = seed;
   cfg.sum_of_distances = 1;
+  cfg.counter_memory = counter_memory;
+  cfg.stream_tile_nodes = tile_nodes;
   hb_instance *hb = nullptr;
   check(hb_create(&g, &t, &cfg, &hb), "hb_create");
   if (heavy) check(hb_set_tuning(hb, heavy), "hb_set_tuning");
   check(hb_set_delta_gathering(hb, delta_mode), "hb_set_delta_gathering");
+  if (counter_memory == HB_COUNTERS_STREAM) check(hb_set_streaming(hb, streaming), "hb_set_streaming");
   or_hb *ref = or_hb_create(n, gg.num_arcs, gg.offsets, gg.successors, gt.offsets, gt.successors,
                             log2m, r, OR_HASH_JENKINS, seed, 1, 0, nullptr, 0, 1);
   if (!ref) {
@@ -60,8 +67,9 @@ int main(int argc, char **argv) {
   check(hb_get_params(hb, &p), "hb_get_params");
   const size_t words = size_t(n) * p.words_per_counter;
   std::vector<uint64_t> c(words), c2(words), mod((n + 63) / 64);
-  std::printf("n=%llu arcs=%llu log2m=%u r=%u heavy=%u\n", (unsigned long long)n,
-              (unsigned long long)gg.num_arcs, log2m, r, heavy);
+  std::printf("n=%llu arcs=%llu log2m=%u r=%u heavy=%u counter memory %u tile %llu\n",
+              (unsigned long long)n, (unsigned long long)gg.num_arcs, log2m, r, heavy,
+              counter_memory, (unsigned long long)tile_nodes);
   check(hb_init(hb), "hb_init");
   or_hb_init(ref);
   int bad = 0;
@@ -83,8 +91,8 @@ int main(int argc, char **argv) {
     const bool same_c = std::memcmp(c.data(), or_hb_counters(ref, 0), words * 8) == 0;
     const bool same_m = std::memcmp(mod.data(), or_hb_modified_bits(ref), mod.size() * 8) == 0;
     const bool same_mode = mode == or_hb_mode(ref);
-    std::printf("iteration %d: mode %d modified %llu delta %u gathered %llu of %llu: %s\n", it,
-                mode, (unsigned long long)st.modified, st.delta_gathering,
+    std::printf("iteration %d: mode %d modified %llu delta %u streamed %u gathered %llu of %llu: %s\n",
+                it, mode, (unsigned long long)st.modified, st.delta_gathering, st.streamed,
                 (unsigned long long)st.gathered_slabs,
                 (unsigned long long)(st.visited_arcs * (p.num_registers / 32)),
                 same_c && same_m && same_mode ? "identical" : "DIFFERENT");

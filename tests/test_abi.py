@@ -63,7 +63,7 @@ def test_builder_word_alignment_error(hbm):
 
 def test_builder_unknown_counter_memory(hbm):
     with pytest.raises(hbm.HyperBallError) as e:
-        hbm.HyperBallBuilder.with_hyper_log_log(_small(hbm), None, 6).counter_memory(2).build()
+        hbm.HyperBallBuilder.with_hyper_log_log(_small(hbm), None, 6).counter_memory(3).build()
     assert e.value.code == hbm.HB_E_INVALID_ARG
     assert "unknown counter memory" in e.value.message
 
@@ -98,7 +98,7 @@ def test_builder_bad_graph(hbm):
 
 def test_unsupported_shape(hbm):
     with pytest.raises(hbm.HyperBallError) as e:
-        hbm.HyperBallBuilder.with_hyper_log_log(_small(hbm), None, 13).build()
+        hbm.HyperBallBuilder.with_hyper_log_log(_small(hbm), None, 17).build()
     assert "no kernel" in e.value.message
 
 

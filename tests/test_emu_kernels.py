@@ -50,6 +50,9 @@ def test_emulated_batch_matches_single_instances(emu_exes, graph, heavy, tune):
     (8, 5, 300, 250, 9, 64, 0),     # 40 dwords, stride 48
     (7, 5, 300, 250, 10, 0, 1),     # 20 dwords, stride 32
     (13, 5, 200, 150, 6, 0, 1),     # two 64-lane segments per counter, delta gathering forced
+    (8, 6, 500, 300, 5, 0, "0 2 128 1"),   # streamed counters: 4 tiles, blocks of 32 nodes
+    (13, 5, 200, 150, 6, 0, "0 2 128 1"),  # streamed, two segments per counter
+    (6, 5, 2000, 300, 9, 0, "0 2 1024"),   # 2 tiles, streamed by the per-iteration rule
 ])
 def test_emulated_engine_matches_oracle(emu_exes, log2m, r, n, dmax, seed, heavy, delta):
     """Every iteration (standard, systolic, local, pre-local) of the emulated
@@ -57,7 +60,7 @@ def test_emulated_engine_matches_oracle(emu_exes, log2m, r, n, dmax, seed, heavy
     restatement of the reference engine: registers, modified bits, mode; then
     sums of distances."""
     args = [emu_exes["oracle_check"], str(log2m), str(r), str(n), str(dmax), str(seed), str(heavy),
-            str(delta)]
+            *str(delta).split()]
     res = subprocess.run(args, capture_output=True, text=True, timeout=900)
     assert res.returncode == 0 and res.stdout.strip().endswith("OK"), res.stdout[-3000:] + res.stderr[-6000:]
     assert "DIFFERENT" not in res.stdout

@@ -320,20 +320,25 @@ def test_progress_callback_and_caller_stream(hbm):
 
 
 def test_host_counter_memory_outputs(hbm):
-    """The same run with HBM and with pinned host counter arrays gives
-    identical outputs; an unknown memory kind is rejected."""
+    """The same run with HBM counter arrays, pinned host ones, and host ones
+    streamed through device memory (several tiles, one tile) gives identical
+    outputs, statistics included; an unknown memory kind is rejected."""
     g, gt = _web(hbm, 20000, seed=12)
     runs = []
-    for where in (hbm.COUNTERS_DEVICE, hbm.COUNTERS_HOST):
+    for where, tile in ((hbm.COUNTERS_DEVICE, 0), (hbm.COUNTERS_HOST, 0),
+                        (hbm.COUNTERS_STREAM, 2048), (hbm.COUNTERS_STREAM, 0)):
         hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).jenkins(4).sum_of_distances(True)
-              .sum_of_inverse_distances(True).counter_memory(where).build())
+              .sum_of_inverse_distances(True).counter_memory(where, tile).build())
         assert hb.params()["counter_memory"] == where
         hb.run_until_done()
-        runs.append((hb.counters(0), hb.sum_of_distances(), hb.harmonic_centralities(),
-                     hb.neighborhood_function()))
+        runs.append((hb.counters(0), hb.counters(1), hb.sum_of_distances(),
+                     hb.harmonic_centralities(), hb.neighborhood_function(),
+                     [(s["checked"], s["modified"], s["visited_arcs"], s["copied"])
+                      for s in hb.iteration_stats()]))
         hb.close()
-    for a, b in zip(*runs):
-        assert np.array_equal(a, b)
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert np.array_equal(a, b)
     with pytest.raises(hbm.HyperBallError) as e:
-        hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).counter_memory(2).build()
+        hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 8).counter_memory(3).build()
     assert e.value.code == hbm.HB_E_INVALID_ARG

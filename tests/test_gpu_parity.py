@@ -28,12 +28,12 @@ def _tiny_graphs(hbm):
 
 
 def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_000, heavy=None,
-                  delta=None, counter_memory=None):
+                  delta=None, counter_memory=None, tile_nodes=0, streaming=None):
     b = hbm.HyperBallBuilder.with_hyper_log_log(g, gt, log2m).register_bits(reg_bits)
     if jenkins is not None:
         b = b.jenkins(jenkins)
     if counter_memory is not None:
-        b = b.counter_memory(counter_memory)
+        b = b.counter_memory(counter_memory, tile_nodes)
     hb = b.build()
     if counter_memory is not None:
         assert hb.params()["counter_memory"] == counter_memory
@@ -41,6 +41,8 @@ def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_
         hb.set_tuning(heavy)
     if delta is not None:
         hb.set_delta_gathering(delta)
+    if streaming is not None:
+        hb.set_streaming(streaming)
     hll = orc.Hll(log2m, reg_bits, g.num_nodes, orc.HASH_JENKINS if jenkins is not None else orc.HASH_SIPHASH13,
                   jenkins or 0)
     ref = orc.Jacobi(hll, g.offsets, g.successors)
@@ -190,3 +192,30 @@ def test_host_counter_memory_bit_exact(hbm, orc, log2m, reg_bits):
     it, stats = _run_lockstep(hbm, orc, g, g.transpose(), log2m, reg_bits, jenkins=2, heavy=128,
                               counter_memory=hbm.COUNTERS_HOST)
     assert it > 5 and any(s["systolic"] for s in stats)
+
+
+@pytest.mark.parametrize("log2m,reg_bits,tile", [(6, 5, 512), (8, 6, 1024), (10, 6, 640),
+                                                 (12, 5, 6016), (13, 5, 768), (8, 5, 0)])
+def test_streamed_counter_memory_bit_exact(hbm, orc, log2m, reg_bits, tile):
+    """Counter arrays in pinned host memory streamed through device memory in
+    tiles (uploaded source blocks merged pass by pass): every iteration
+    bit-exact against the Jacobi oracle, with systolic, local and pre-local
+    iterations, many tiles and blocks, one tile (6016 >= n), counters of two
+    segments, and the automatic tile size."""
+    g = hbm.Graph.web(6000, 2.1, 1, 1500, 100, 0.7, 3 + log2m)
+    it, stats = _run_lockstep(hbm, orc, g, g.transpose(), log2m, reg_bits, jenkins=2,
+                              counter_memory=hbm.COUNTERS_STREAM, tile_nodes=tile, streaming=1)
+    assert it > 5 and any(s["systolic"] for s in stats)
+    assert any(s["local"] for s in stats)
+    assert all(s["streamed"] for s in stats)
+
+
+def test_streamed_counter_memory_rule_bit_exact(hbm, orc):
+    """The per-iteration rule of streamed counters: the seeded first
+    iteration and the sparse late ones read the host arrays in place, the
+    dense ones stream them (2 tiles); every iteration bit-exact."""
+    g = hbm.Graph.web(20000, 2.1, 3, 1500, 200, 0.7, 8)
+    it, stats = _run_lockstep(hbm, orc, g, g.transpose(), 8, 5, jenkins=3,
+                              counter_memory=hbm.COUNTERS_STREAM, tile_nodes=10016)
+    streamed = [s["streamed"] for s in stats]
+    assert streamed[0] == 0 and streamed[-1] == 0 and any(streamed), streamed

@@ -157,13 +157,38 @@ struct Instance {
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
+  // Streamed counters (HB_COUNTERS_STREAM): both arrays in pinned host
+  // memory; an iteration walks tiles of tile_nodes nodes.  Per tile: upload
+  // its own counters (s_own[2], alternately per tile, so the next tile's
+  // upload need not wait for this tile's passes), merge them and then every
+  // uploaded block (block_nodes nodes, s_src[2] alternately) that holds
+  // modified nodes into the partial next counters (s_part[2], alternately per
+  // tile), finish, download.  Uploads run on s_up, downloads on s_down,
+  // kernels on `stream`; events order the buffer reuse.
+  uint32_t counter_memory = HB_COUNTERS_DEVICE;
+  uint64_t tile_nodes = 0, block_nodes = 0, tiles = 0, blocks = 0;
+  DevBuf<uint32_t> s_own[2], s_part[2], s_src[2], s_counts;
+  uint32_t *h_counts = nullptr;  // pinned: per tile the nodes to work on, per block the modified
+  hipStream_t s_up = nullptr, s_down = nullptr;
+  enum { kOwnReady0, kOwnReady1, kOwnFree0, kOwnFree1, kSrcReady0, kSrcReady1, kSrcFree0,
+         kSrcFree1, kPartDone0, kPartDone1, kPartFree0, kPartFree1, kPreReady, kOtherRead,
+         kDownDone, kStreamEvents };
+  hipEvent_t sev[kStreamEvents] = {};
+
   ~Instance() {
     if (stream) {
       (void)hipSetDevice(device);
       (void)hipStreamSynchronize(stream);
     }
+    for (auto q : {s_up, s_down})
+      if (q) (void)hipStreamSynchronize(q);
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto &e : sev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto q : {s_up, s_down})
+      if (q) (void)hipStreamDestroy(q);
+    if (h_counts) (void)hipHostFree(h_counts);
     if (h_stats) (void)hipHostFree(h_stats);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
@@ -396,6 +421,7 @@ struct Instance {
     hs.modified = st.modified;
     hs.gathered_slabs = st.gathered;
     hs.delta_gathering = delta;
+    hs.streamed = streamed_now;
     last_changed = st.changed;
     hs.union_ms = union_ms;
     hs.iteration_ms = iteration_ms;
@@ -416,7 +442,155 @@ struct Instance {
     if (progress) progress(&history.back(), progress_ctx);
   }
 
+  // HyperBall::iterate with streamed counters: the same plan and per-node
+  // body as iterate(), worked tile by tile (see the streamed-counter fields).
+  // A tile with no node to check and no node modified last iteration is
+  // skipped: its next counters already equal its current ones in the other
+  // host array.  Source blocks without modified nodes are not uploaded.
+  void iterate_streamed() {
+    use_device();
+    Plan p = plan_iteration();
+    p.ua.flags &= ~(F_SEEDED | F_DELTA);  // counters are read, whole, from the uploaded ranges
+    delta = false;
+    if (timing) HB_HIP(hipEventRecord(ev[0], stream));
+    HB_HIP(launch_prepare_iteration(p.ua.next_mod, systolic ? p.ua.next_must : nullptr,
+                                    bitmap_words, stats.p, work.p, stat_slots.p, stream));
+    if (p.mark_first) {
+      fill_zero(must[mustcur].p, bitmap_words * 4);
+      HB_HIP(launch_compact(mod[mcur].p, nullptr, n, bsum.p, list.p, list_len.p, stream));
+      HB_HIP(launch_mark_predecessors(list.p, list_len.p, toff.p, tsucc.p, must[mustcur].p,
+                                      max_grid, stream));
+    }
+    fill_zero(s_counts.p, (tiles + blocks) * 4);
+    HB_HIP(launch_stream_summary(p.ua, n, tile_nodes, block_nodes, tiles, s_counts.p, stream));
+    HB_HIP(hipMemcpyAsync(h_counts, s_counts.p, (tiles + blocks) * 4, hipMemcpyDeviceToHost,
+                          stream));
+    HB_HIP(hipStreamSynchronize(stream));
+    if (timing) HB_HIP(hipEventRecord(ev[1], stream));
+    const uint64_t bytes_per_node = uint64_t(cstride) * 4;
+    const uint32_t *host_cur = cnt[cur].p;
+    uint32_t *host_next = cnt[1 - cur].p;
+    const uint64_t bpt = tile_nodes / block_nodes;  // blocks per tile
+    std::vector<uint64_t> work_tiles;
+    for (uint64_t t = 0; t < tiles; t++)
+      if (h_counts[t]) work_tiles.push_back(t);
+    // Which tile each own buffer holds this iteration, and which of its
+    // blocks have arrived.  A tile's blocks that the previous tile needed as
+    // sources were uploaded straight into its own buffer; blocks of the
+    // previous tile are read from the other own buffer while it still holds
+    // them.
+    int64_t held[2] = {-1, -1};
+    std::vector<char> loaded[2] = {std::vector<char>(bpt, 0), std::vector<char>(bpt, 0)};
+    auto upload = [&](uint32_t *dst, uint64_t lo, uint64_t hi) {
+      HB_HIP(hipMemcpyAsync(dst, host_cur + lo * cstride, (hi - lo) * bytes_per_node,
+                            hipMemcpyHostToDevice, s_up));
+    };
+    int pi = 0, k = 0;
+    for (size_t w = 0; w < work_tiles.size(); w++) {
+      const uint64_t t = work_tiles[w];
+      const int64_t next_tile = w + 1 < work_tiles.size() ? int64_t(work_tiles[w + 1]) : -1;
+      const uint64_t tl = t * tile_nodes, th = std::min(n, tl + tile_nodes);
+      const int other = 1 - pi;
+      if (held[pi] != int64_t(t)) {
+        HB_HIP(hipStreamWaitEvent(s_up, sev[kOwnFree0 + pi], 0));
+        held[pi] = int64_t(t);
+        std::fill(loaded[pi].begin(), loaded[pi].end(), 0);
+      }
+      for (uint64_t j = 0; j < bpt && tl + j * block_nodes < th; j++) {
+        if (loaded[pi][j]) continue;
+        const uint64_t bl = tl + j * block_nodes, bh = std::min(th, bl + block_nodes);
+        upload(s_own[pi].p + (bl - tl) * cstride, bl, bh);
+        loaded[pi][j] = 1;
+      }
+      HB_HIP(hipEventRecord(sev[kOwnReady0 + pi], s_up));
+      HB_HIP(hipStreamWaitEvent(stream, sev[kOwnReady0 + pi], 0));
+      HB_HIP(hipStreamWaitEvent(stream, sev[kPartFree0 + pi], 0));
+      StreamPass sp{s_own[pi].p, tl, th, s_own[pi].p, s_part[pi].p, tl, th, 1};
+      HB_HIP(launch_stream_merge(int(r), int(log2m), p.ua, sp, stream));
+      sp.first = 0;
+      bool other_read = false;  // passes of this tile read the other own buffer
+      for (uint64_t b = 0; b < blocks; b++) {
+        const uint64_t bl = b * block_nodes, bh = std::min(n, bl + block_nodes);
+        if ((bl >= tl && bh <= th) || h_counts[tiles + b] == 0) continue;
+        const int64_t bt = int64_t(b / bpt);
+        const uint64_t j = b % bpt, off = j * block_nodes * cstride;
+        sp.lo = bl;
+        sp.hi = bh;
+        if (held[other] == bt && loaded[other][j]) {
+          sp.src = s_own[other].p + off;
+          other_read = true;
+        } else if (bt == next_tile) {
+          if (held[other] != bt) {
+            if (other_read) {
+              HB_HIP(hipEventRecord(sev[kOtherRead], stream));
+              HB_HIP(hipStreamWaitEvent(s_up, sev[kOtherRead], 0));
+              other_read = false;
+            }
+            HB_HIP(hipStreamWaitEvent(s_up, sev[kOwnFree0 + other], 0));
+            held[other] = bt;
+            std::fill(loaded[other].begin(), loaded[other].end(), 0);
+          }
+          upload(s_own[other].p + off, bl, bh);
+          loaded[other][j] = 1;
+          HB_HIP(hipEventRecord(sev[kPreReady], s_up));
+          HB_HIP(hipStreamWaitEvent(stream, sev[kPreReady], 0));
+          sp.src = s_own[other].p + off;
+        } else {
+          HB_HIP(hipStreamWaitEvent(s_up, sev[kSrcFree0 + k], 0));
+          upload(s_src[k].p, bl, bh);
+          HB_HIP(hipEventRecord(sev[kSrcReady0 + k], s_up));
+          HB_HIP(hipStreamWaitEvent(stream, sev[kSrcReady0 + k], 0));
+          sp.src = s_src[k].p;
+          HB_HIP(launch_stream_merge(int(r), int(log2m), p.ua, sp, stream));
+          HB_HIP(hipEventRecord(sev[kSrcFree0 + k], stream));
+          k ^= 1;
+          continue;
+        }
+        HB_HIP(launch_stream_merge(int(r), int(log2m), p.ua, sp, stream));
+      }
+      HB_HIP(launch_stream_finish(int(r), int(log2m), p.ua, sp, stream));
+      // nothing issued so far reads either own buffer after this point
+      HB_HIP(hipEventRecord(sev[kOwnFree0], stream));
+      HB_HIP(hipEventRecord(sev[kOwnFree1], stream));
+      HB_HIP(hipEventRecord(sev[kPartDone0 + pi], stream));
+      HB_HIP(hipStreamWaitEvent(s_down, sev[kPartDone0 + pi], 0));
+      HB_HIP(hipMemcpyAsync(host_next + tl * cstride, s_part[pi].p, (th - tl) * bytes_per_node,
+                            hipMemcpyDeviceToHost, s_down));
+      HB_HIP(hipEventRecord(sev[kPartFree0 + pi], s_down));
+      pi = other;
+    }
+    HB_HIP(hipEventRecord(sev[kDownDone], s_down));
+    HB_HIP(hipStreamWaitEvent(stream, sev[kDownDone], 0));
+    if (timing) HB_HIP(hipEventRecord(ev[2], stream));
+    HB_HIP(launch_reduce_stats(stats.p, stat_slots.p, h_stats, stream));
+    if (timing) HB_HIP(hipEventRecord(ev[3], stream));
+    HB_HIP(hipStreamSynchronize(stream));
+    float ums = 0, ims = 0;
+    if (timing) {
+      HB_HIP(hipEventElapsedTime(&ums, ev[1], ev[2]));
+      HB_HIP(hipEventElapsedTime(&ims, ev[0], ev[3]));
+    }
+    finish_iteration(p, *h_stats, ums, ims);
+  }
+
+  // Streamed counters: whether this iteration streams them (hb_set_streaming).
+  int stream_mode = 0;
+  bool streamed_now = false;  // the iteration under way streams them
+  bool stream_this_iteration() const {
+    if (counter_memory != HB_COUNTERS_STREAM || stream_mode == 2) return false;
+    if (stream_mode == 1) return true;
+    if (iteration == 0) return weighted;  // seeded registers come from hashes: nothing to read
+    const double visited = double(modified) * double(a) / double(std::max<uint64_t>(n, 1));
+    const double streamed = double(n) * double(tiles > 3 ? tiles - 2 : 1);
+    return visited >= streamed;
+  }
+
   void iterate() {
+    streamed_now = stream_this_iteration();
+    if (streamed_now) {
+      iterate_streamed();
+      return;
+    }
     use_device();
     Plan p = plan_iteration();
     if (timing) HB_HIP(hipEventRecord(ev[0], stream));
@@ -765,7 +939,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     }
     if (cfg->acc_precision > HB_ACC_F32) fail(HB_E_INVALID_ARG, "unknown accumulator precision");
     if (cfg->hash > HB_HASH_JENKINS) fail(HB_E_INVALID_ARG, "unknown hash");
-    if (cfg->counter_memory > HB_COUNTERS_HOST) fail(HB_E_INVALID_ARG, "unknown counter memory");
+    if (cfg->counter_memory > HB_COUNTERS_STREAM) fail(HB_E_INVALID_ARG, "unknown counter memory");
     uint64_t num_elements = cfg->num_elements ? cfg->num_elements : n;
     if (cfg->weights && !cfg->num_elements) {
       num_elements = 0;
@@ -881,7 +1055,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
 
     I.bitmap_words = std::max<uint64_t>(2, ((n + 63) / 64) * 2);
     for (int k = 0; k < 2; k++) {
-      I.cnt[k].alloc(size_t(n) * I.cstride, cfg->counter_memory == HB_COUNTERS_HOST);
+      I.cnt[k].alloc(size_t(n) * I.cstride, cfg->counter_memory != HB_COUNTERS_DEVICE);
       I.mod[k].alloc(I.bitmap_words);
       I.must[k].alloc(I.bitmap_words);
       I.chg[k].alloc(n);
@@ -912,6 +1086,31 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     I.stats.alloc(1);
     I.stat_slots.alloc(size_t(kStatSlots) * kStatSlotWords);
     I.work.alloc(kWorkCounters * kWorkStride);
+    I.counter_memory = cfg->counter_memory;
+    if (I.counter_memory == HB_COUNTERS_STREAM) {
+      const uint64_t node_bytes = uint64_t(I.cstride) * 4;
+      uint64_t t = cfg->stream_tile_nodes ? cfg->stream_tile_nodes
+                                          : (uint64_t(1) << 30) / node_bytes;
+      t = std::min<uint64_t>(std::max<uint64_t>(t, 128), (n + 127) / 128 * 128);
+      t = (t + 127) / 128 * 128;
+      I.tile_nodes = t;
+      I.block_nodes = t / 4;  // a multiple of 32, as the summary kernel needs
+      I.tiles = (n + t - 1) / t;
+      I.blocks = (n + I.block_nodes - 1) / I.block_nodes;
+      for (int k = 0; k < 2; k++) {
+        I.s_own[k].alloc(t * I.cstride);
+        I.s_part[k].alloc(t * I.cstride);
+        I.s_src[k].alloc(I.block_nodes * I.cstride);
+      }
+      I.s_counts.alloc(I.tiles + I.blocks);
+      void *hc = nullptr;
+      HB_HIP(hipHostMalloc(&hc, std::max<uint64_t>(1, I.tiles + I.blocks) * 4,
+                           hipHostMallocDefault));
+      I.h_counts = static_cast<uint32_t *>(hc);
+      HB_HIP(hipStreamCreateWithFlags(&I.s_up, hipStreamNonBlocking));
+      HB_HIP(hipStreamCreateWithFlags(&I.s_down, hipStreamNonBlocking));
+      for (auto &e : I.sev) HB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     HB_HIP(hipStreamSynchronize(s));
     return inst;
 }
@@ -1130,7 +1329,7 @@ hb_status hb_get_params(hb_instance *hb, hb_params *out) {
     out->num_nodes = I.n;
     out->num_arcs = I.a;
     out->alpha_m_m = I.amm;
-    out->counter_memory = I.cnt[0].host ? HB_COUNTERS_HOST : HB_COUNTERS_DEVICE;
+    out->counter_memory = I.counter_memory;
   });
 }
 
@@ -1256,6 +1455,13 @@ hb_status hb_set_delta_gathering(hb_instance *hb, int mode) {
   });
 }
 
+hb_status hb_set_streaming(hb_instance *hb, int mode) {
+  return guard([&] {
+    if (mode < 0 || mode > 2) fail(HB_E_INVALID_ARG, "streaming mode must be 0, 1 or 2");
+    checked(hb)->stream_mode = mode;
+  });
+}
+
 hb_status hb_autotune(hb_instance *hb, uint32_t reps, uint32_t *thresholds, double *ms,
                       size_t cap, size_t *len, uint32_t *chosen) {
   return guard([&] {
@@ -1328,6 +1534,9 @@ hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transpos
     b->device = cfgs[0].device;
     for (size_t i = 0; i < k; i++) {
       if (cfgs[i].device != b->device) fail(HB_E_INVALID_ARG, "all instances of a batch must use one device");
+      if (cfgs[i].counter_memory == HB_COUNTERS_STREAM)
+        fail(HB_E_INVALID_ARG, "instance " + std::to_string(i) +
+                                   ": streamed counters are not available in a batch");
       const uint32_t r = cfgs[i].reg_bits ? cfgs[i].reg_bits : 5u;
       if (batch_shape(int(r), int(cfgs[i].log2m)) < 0)
         fail(HB_E_INVALID_ARG, "instance " + std::to_string(i) +

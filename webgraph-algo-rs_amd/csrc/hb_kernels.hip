@@ -1414,6 +1414,166 @@ void batch_union_kernel(const BatchDesc *d, const uint32_t *inst, int cnt,
   if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
 }
 
+// ---- streamed counters ------------------------------------------------------
+// The counter arrays stay in host memory; an iteration walks the nodes tile by
+// tile.  For each tile the host uploads the tile's own counters, then every
+// source block that holds modified nodes; one merge pass per source range
+// (the tile itself first) merges the modified successors inside that range
+// into the partial next counters of the tile's checked nodes; a finish pass
+// does the rest of the per-node body and leaves the tile's next counters for
+// the download.  Registers do not depend on the order in which a node's
+// successors are merged (a per-register maximum), so the passes give the
+// union kernel's registers exactly.
+
+// Whether node v is checked this iteration (the union kernel's rule: in list
+// mode the list holds must | modified, and F_CHECK_ALL checks all of it).
+__device__ __forceinline__ bool stream_checked(const UnionArgs &a, uint32_t v, bool was_mod) {
+  const bool must = test_bit(a.must, v);
+  if (a.flags & F_LIST) return (a.flags & F_CHECK_ALL) ? (must || was_mod) : must;
+  return (a.flags & F_CHECK_ALL) || must;
+}
+
+// First position in succ[b, e) (sorted) holding a value >= x.
+__device__ __forceinline__ uint64_t row_lower_bound(const uint32_t *succ, uint64_t b, uint64_t e,
+                                                    uint64_t x) {
+  while (b < e) {
+    const uint64_t mid = b + (e - b) / 2;
+    if (succ[mid] < x) b = mid + 1;
+    else e = mid;
+  }
+  return b;
+}
+
+template <int R, int G, int SPL, int NSEG>
+__global__ __launch_bounds__(kBlock) void stream_merge_kernel(UnionArgs a, StreamPass p) {
+  constexpr int D = R * SPL, CD = G * D, K = ids_per_lane<G>(), SW = slice_width<G>();
+  constexpr int NG = kBlock / G, U = D <= 6 ? 8 : D <= 12 ? 4 : 2;
+  constexpr uint32_t CS = counter_stride(NSEG * CD);
+  __shared__ uint32_t s_ids[NG * SW];
+  const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  uint32_t *ids = s_ids + g * SW;
+  const uint64_t below = (uint64_t(1) << gl) - 1;
+  Acc acc;
+  const uint64_t nt = p.tile_hi - p.tile_lo;
+  for (uint64_t i = uint64_t(blockIdx.x) * NG + g; i < nt; i += uint64_t(gridDim.x) * NG) {
+    const uint32_t v = uint32_t(p.tile_lo + i);
+    if (!stream_checked(a, v, test_bit(a.curr_mod, v))) continue;
+    const uint64_t rb = a.off[v], re = a.off[v + 1];
+    const uint64_t b = row_lower_bound(a.succ, rb, re, p.lo);
+    const uint64_t e = row_lower_bound(a.succ, b, re, p.hi);
+    uint32_t nvisit = 0;
+#pragma unroll 1
+    for (int seg = 0; seg < NSEG; seg++) {
+      uint32_t cur[D];
+      const uint64_t at = i * CS + seg * CD + gl * D;
+      load_part<D>(cur, (p.first ? p.own : p.part) + at);
+      bool any = false;
+      const uint32_t *lane_base = p.src + seg * CD + gl * D;
+      for (uint64_t base = b; base < e; base += SW) {
+        uint32_t sj[K];
+        bool f[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+          const uint64_t idx = base + uint64_t(j) * G + gl;
+          sj[j] = idx < e ? a.succ[idx] : v;
+          f[j] = sj[j] != v && test_bit(a.curr_mod, sj[j]);
+        }
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+          const uint64_t mask = group_ballot<G>(f[j]);
+          if (f[j]) ids[cnt + __popcll(mask & below)] = sj[j];
+          cnt += __popcll(mask);
+        }
+        if (!cnt) continue;
+        lane_sync<G>();
+        any = true;
+        if (seg == 0) nvisit += cnt;
+        acc.gathered += uint64_t(cnt) * SPL;
+        for (uint32_t k = 0; k < cnt; k += U) {
+          uint32_t y[U][D];
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const uint32_t t = ids[k + u < cnt ? k + u : k];
+            load_part<D>(y[u], lane_base + uint64_t(t - p.lo) * CS);
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < SPL; q++) slab_max<R>(cur + q * R, y[u] + q * R);
+        }
+        lane_sync<G>();
+      }
+      if (p.first || any) store_part<D>(p.part + at, cur);
+    }
+    if (gl == 0) acc.visited += nvisit;
+  }
+  flush_stats(acc, a.stats, a.stat_slots);
+}
+
+template <int R, int G, int SPL, int NSEG>
+__global__ __launch_bounds__(kBlock) void stream_finish_kernel(UnionArgs a, StreamPass p) {
+  constexpr int D = R * SPL, CD = G * D, NG = kBlock / G;
+  constexpr uint32_t CS = counter_stride(NSEG * CD);
+  const int g = threadIdx.x / G, gl = threadIdx.x % G;
+  Acc acc;
+  const uint64_t nt = p.tile_hi - p.tile_lo;
+  for (uint64_t i = uint64_t(blockIdx.x) * NG + g; i < nt; i += uint64_t(gridDim.x) * NG) {
+    const uint32_t v = uint32_t(p.tile_lo + i);
+    const bool was_mod = test_bit(a.curr_mod, v);
+    const bool check = stream_checked(a, v, was_mod);
+    int64_t s5 = 0;
+    uint64_t hi = 0, lo = 0;
+    uint32_t zeros = 0, chg_bits = 0;
+#pragma unroll 1
+    for (int seg = 0; seg < NSEG; seg++) {
+      const uint64_t at = i * CS + seg * CD + gl * D;
+      uint32_t prev[D];
+      load_part<D>(prev, p.own + at);
+      if (check) {
+        uint32_t cur[D];
+        load_part<D>(cur, p.part + at);
+#pragma unroll
+        for (int q = 0; q < SPL; q++) {
+          uint32_t d = 0;
+#pragma unroll
+          for (int b = 0; b < R; b++) d |= cur[q * R + b] ^ prev[q * R + b];
+          if (d) chg_bits |= 1u << ((uint32_t(seg) * G * SPL + uint32_t(gl) * SPL + q) >> a.chg_shift);
+        }
+        lane_sums<R, SPL>(cur, s5, hi, lo, zeros);
+      } else {
+        store_part<D>(p.part + at, prev);  // the tile's next counters are downloaded whole
+      }
+    }
+    if (!check && !was_mod) continue;
+    const bool cm = check && group_ballot<G>(chg_bits != 0) != 0;
+    double post = 0.0;
+    if (cm) {
+      chg_bits = group_or_u32<G>(chg_bits);
+      post = group_estimate<R, G>(s5, hi, lo, zeros, a.alpha_m_m, a.m, a.lc_table);
+    }
+    node_update<G>(a, v, gl, check, cm, chg_bits, post, 0u,
+                   check ? a.off[v + 1] - a.off[v] : 0, acc);
+  }
+  flush_stats(acc, a.stats, a.stat_slots);
+}
+
+__global__ __launch_bounds__(kBlock) void stream_summary_kernel(UnionArgs a, uint64_t n,
+                                                                uint64_t tile_nodes,
+                                                                uint64_t block_nodes,
+                                                                uint64_t tiles, uint32_t *counts) {
+  const uint64_t words = (n + 31) / 32;
+  const bool all = (a.flags & F_CHECK_ALL) && !(a.flags & F_LIST);
+  for (uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x; w < words;
+       w += uint64_t(gridDim.x) * kBlock) {
+    const uint32_t valid = (w == words - 1 && (n & 31)) ? (1u << (n & 31)) - 1 : ~0u;
+    const uint32_t mod = a.curr_mod[w] & valid;
+    const uint32_t work = all ? valid : (a.must[w] | mod) & valid;
+    if (work) atomicAdd(counts + (w * 32) / tile_nodes, uint32_t(__popc(work)));
+    if (mod) atomicAdd(counts + tiles + (w * 32) / block_nodes, uint32_t(__popc(mod)));
+  }
+}
+
 uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap) {
   uint64_t g = (items + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -1631,6 +1791,46 @@ hipError_t launch_mark_predecessors(const uint32_t *list, const uint32_t *len, c
                                    const uint32_t *tsucc, uint32_t *must, uint32_t grid,
                                    hipStream_t s) {
   mark_predecessors_kernel<<<grid, kBlock, 0, s>>>(list, len, toff, tsucc, must);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_merge(int r, int log2m, const UnionArgs &a, const StreamPass &p,
+                               hipStream_t s) {
+  const uint64_t nt = p.tile_hi - p.tile_lo;
+  if (nt == 0) return hipSuccess;
+#define X(RR, LM, GG, SS, NS)                                                                \
+  if (r == RR && log2m == LM) {                                                              \
+    stream_merge_kernel<RR, GG, SS, NS><<<grid_for(nt, kBlock / GG, 16384), kBlock, 0, s>>>(a, \
+                                                                                       p);   \
+    return hipGetLastError();                                                                \
+  }
+  HB_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_stream_finish(int r, int log2m, const UnionArgs &a, const StreamPass &p,
+                                hipStream_t s) {
+  const uint64_t nt = p.tile_hi - p.tile_lo;
+  if (nt == 0) return hipSuccess;
+#define X(RR, LM, GG, SS, NS)                                                                 \
+  if (r == RR && log2m == LM) {                                                               \
+    stream_finish_kernel<RR, GG, SS, NS><<<grid_for(nt, kBlock / GG, 16384), kBlock, 0, s>>>(a, \
+                                                                                        p);   \
+    return hipGetLastError();                                                                 \
+  }
+  HB_SHAPES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_stream_summary(const UnionArgs &a, uint64_t n, uint64_t tile_nodes,
+                                 uint64_t block_nodes, uint64_t tiles, uint32_t *counts,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (tile_nodes % 32 || block_nodes % 32) return hipErrorInvalidValue;
+  stream_summary_kernel<<<grid_for((n + 31) / 32, kBlock, 4096), kBlock, 0, s>>>(
+      a, n, tile_nodes, block_nodes, tiles, counts);
   return hipGetLastError();
 }
 

@@ -125,6 +125,20 @@ struct UnionArgs {
   uint32_t chg_shift;
 };
 
+// Streamed counters (counter arrays in host memory, worked on tile by tile
+// in device memory).  One pass merges, into the partial next counters of the
+// tile's checked nodes, their modified successors inside one source range
+// [lo, hi) whose counters sit in `src`; the first pass of a tile also
+// initialises the partial counters from the tile's own counters.
+struct StreamPass {
+  const uint32_t *src;  // counters of nodes lo .. hi - 1, at the counter stride
+  uint64_t lo, hi;
+  const uint32_t *own;  // current counters of the tile's nodes
+  uint32_t *part;       // partial next counters of the tile's nodes
+  uint64_t tile_lo, tile_hi;
+  uint32_t first;
+};
+
 struct SeedArgs {
   uint32_t *counters;  // zeroed, slab layout
   uint64_t n;
@@ -223,6 +237,21 @@ hipError_t launch_mark_predecessors(const uint32_t *list, const uint32_t *len, c
                                    const uint32_t *tsucc, uint32_t *must, uint32_t grid,
                                    hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s);
+// Streamed counters.  launch_stream_merge: one StreamPass over the tile's
+// checked nodes.  launch_stream_finish: the rest of the per-node body for
+// every node of the tile (changed test, count(), bookkeeping) with the
+// complete next counters of the tile left in p.part (unchecked nodes get a
+// copy of their own counters).  launch_stream_summary: counts[t] = nodes of
+// tile t that are checked or were modified, counts[tiles + b] = modified nodes
+// of block b (tiles of tile_nodes nodes, blocks of block_nodes nodes); counts
+// must be zeroed.
+hipError_t launch_stream_merge(int r, int log2m, const UnionArgs &a, const StreamPass &p,
+                               hipStream_t s);
+hipError_t launch_stream_finish(int r, int log2m, const UnionArgs &a, const StreamPass &p,
+                                hipStream_t s);
+hipError_t launch_stream_summary(const UnionArgs &a, uint64_t n, uint64_t tile_nodes,
+                                 uint64_t block_nodes, uint64_t tiles, uint32_t *counts,
+                                 hipStream_t s);
 // Transpose of the CSR graph (off[n+1], succ[a]) into caller-allocated
 // toff[n+1], tsucc[a], rows sorted (hb_transpose.hip).  Allocates and frees
 // its own temporaries (about 8 bytes per arc plus the sort's workspace) and

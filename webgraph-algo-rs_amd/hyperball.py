@@ -31,7 +31,7 @@ LIB_PATH = os.path.join(_HERE, "libhyperball_amd.so")
 HB_OK, HB_E_NOT_RUN, HB_E_NOT_REQUESTED, HB_E_INVALID_ARG, HB_E_OOM, HB_E_HIP, HB_E_NO_DEVICE = range(7)
 HASH_SIPHASH13, HASH_JENKINS = 0, 1
 ACC_F64, ACC_F32_KAHAN, ACC_F32 = 0, 1, 2
-COUNTERS_DEVICE, COUNTERS_HOST = 0, 1
+COUNTERS_DEVICE, COUNTERS_HOST, COUNTERS_STREAM = 0, 1, 2
 
 
 class HyperBallError(RuntimeError):
@@ -59,7 +59,7 @@ class _Config(ctypes.Structure):
                 ("arc_granularity", ctypes.c_uint64), ("sum_of_distances", ctypes.c_int),
                 ("sum_of_inverse_distances", ctypes.c_int), ("acc_precision", ctypes.c_uint32),
                 ("device", ctypes.c_int), ("device_transpose", ctypes.c_int),
-                ("counter_memory", ctypes.c_uint32)]
+                ("counter_memory", ctypes.c_uint32), ("stream_tile_nodes", ctypes.c_uint64)]
 
 
 class _Params(ctypes.Structure):
@@ -76,7 +76,8 @@ class IterationStats(ctypes.Structure):
                 ("scanned_arcs", ctypes.c_uint64), ("transpose_arcs", ctypes.c_uint64),
                 ("copied", ctypes.c_uint64), ("modified", ctypes.c_uint64), ("nf", ctypes.c_double),
                 ("union_ms", ctypes.c_double), ("iteration_ms", ctypes.c_double),
-                ("gathered_slabs", ctypes.c_uint64), ("delta_gathering", ctypes.c_uint32)]
+                ("gathered_slabs", ctypes.c_uint64), ("delta_gathering", ctypes.c_uint32),
+                ("streamed", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -136,6 +137,7 @@ _SIGNATURES = [
     ("hb_batch_set_graph", _I, [_P, _I]),
     ("hb_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
     ("hb_set_delta_gathering", _I, [_P, _I]),
+    ("hb_set_streaming", _I, [_P, _I]),
     ("hb_set_progress", _I, [_P, _P, _P]),
     ("hb_set_stream", _I, [_P, _P]),
     ("hb_batch_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
@@ -298,6 +300,7 @@ class HyperBallBuilder:
         self._device = 0
         self._device_transpose = False
         self._counter_memory = COUNTERS_DEVICE
+        self._stream_tile_nodes = 0
         self._weight_seed = 0
         self._num_elements = 0
         if weights is not None:
@@ -378,12 +381,15 @@ class HyperBallBuilder:
         self._device_transpose = bool(on)
         return self
 
-    def counter_memory(self, where: int) -> "HyperBallBuilder":
+    def counter_memory(self, where: int, tile_nodes: int = 0) -> "HyperBallBuilder":
         """Where the two counter arrays live (the reference's mmap_options
-        argument, hyperball_impl.rs:89-117): COUNTERS_DEVICE (HBM, default) or
+        argument, hyperball_impl.rs:89-117): COUNTERS_DEVICE (HBM, default),
         COUNTERS_HOST (pinned host memory read and written by the kernels
-        over the host link, for counters beyond HBM)."""
+        over the host link) or COUNTERS_STREAM (pinned host memory streamed
+        through device memory in tiles of tile_nodes nodes, 0: automatic),
+        for counters beyond HBM."""
         self._counter_memory = int(where)
+        self._stream_tile_nodes = int(tile_nodes)
         return self
 
     def accumulator_precision(self, mode: int) -> "HyperBallBuilder":
@@ -411,6 +417,7 @@ class HyperBallBuilder:
         cfg.device = self._device
         cfg.device_transpose = int(self._device_transpose)
         cfg.counter_memory = self._counter_memory
+        cfg.stream_tile_nodes = self._stream_tile_nodes
         return cfg
 
     def build(self) -> "HyperBall":
@@ -575,6 +582,11 @@ class HyperBall:
     def set_delta_gathering(self, mode: int) -> None:
         """0: per iteration by the library's rule, 1: always, 2: never."""
         _check(lib().hb_set_delta_gathering(self._h, int(mode)))
+
+    def set_streaming(self, mode: int) -> None:
+        """Streamed counters: 0 by the per-iteration rule, 1 always streamed,
+        2 always read in place over the host link (hb_set_streaming)."""
+        _check(lib().hb_set_streaming(self._h, int(mode)))
 
     def set_tuning(self, heavy_degree: int) -> None:
         """Rows longer than heavy_degree go to the workgroup-per-node kernel
