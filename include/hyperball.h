@@ -77,6 +77,31 @@ hb_status hb_bvgraph_load(const char *basename, hb_graph *out);
  * One thread: a single sequential pass, no .offsets needed. */
 hb_status hb_bvgraph_load_threads(const char *basename, unsigned threads, hb_graph *out);
 
+/* Writes a graph in CSR form (rows strictly increasing) as a BVGraph,
+ * version 0, big-endian: <basename>.graph, .offsets and .properties, in the
+ * published format that hb_bvgraph_load and the reference's
+ * BvGraph::with_basename(..).load() (webgraph 0.2.0) read.  No counterpart in
+ * the reference, which only loads graphs: it stores synthetic graphs and is
+ * the writer of the round-trip tests.  For each node it picks the reference
+ * within the window (at most max_ref_count references deep) whose record
+ * takes the fewest bits. */
+#define HB_CODE_GAMMA 0
+#define HB_CODE_DELTA 1
+#define HB_CODE_UNARY 2
+#define HB_CODE_ZETA 3
+typedef struct {
+  uint32_t window_size;         /* 0: no references */
+  uint32_t max_ref_count;
+  uint32_t min_interval_length; /* 0: no intervals */
+  uint32_t zeta_k;              /* parameter of the zeta codes, 1..16 */
+  uint32_t outdegrees, references, blocks, block_count, residuals, offsets; /* HB_CODE_* */
+} hb_bvgraph_params;
+/* The defaults of the Java BVGraph compressor: window 7, max_ref_count 3,
+ * intervals of at least 4, zeta_3 residuals, gamma elsewhere, unary
+ * references. */
+void hb_bvgraph_params_init(hb_bvgraph_params *p);
+hb_status hb_bvgraph_store(const hb_csr *g, const char *basename, const hb_bvgraph_params *p);
+
 /* Decodes <basename>.offsets (gamma-coded bit-offset gaps) into num_nodes+1
  * bit offsets; used to check a decode against the offsets the writer
  * recorded. */

@@ -406,6 +406,250 @@ void decode_bvgraph_parallel(const std::string &basename, unsigned threads,
   offsets[n] = base;
 }
 
+// ------------------------------------------------------------------------
+// BVGraph writer: the same format, MSB-first bit streams, with the
+// compressor's choices of the published format: for each node the reference
+// within the window (at most maxrefcount references deep) whose encoding
+// takes the fewest bits, ties to the nearer candidate, no reference counted as
+// distance 0; copy blocks over the reference's list; maximal runs of
+// consecutive extra successors of at least min_interval as intervals; the
+// rest as residual gaps.
+// ------------------------------------------------------------------------
+class BitWriter {
+ public:
+  void write_int(uint64_t v, unsigned len) {  // the low `len` bits of v, MSB first
+    for (unsigned i = len; i-- > 0;) put((v >> i) & 1);
+  }
+  void write_unary(uint64_t x) {
+    for (uint64_t i = 0; i < x; i++) put(0);
+    put(1);
+  }
+  void write_gamma(uint64_t x) {
+    const uint64_t y = x + 1;
+    const unsigned msb = 63 - __builtin_clzll(y);
+    write_unary(msb);
+    write_int(y, msb);
+  }
+  void write_delta(uint64_t x) {
+    const uint64_t y = x + 1;
+    const unsigned msb = 63 - __builtin_clzll(y);
+    write_gamma(msb);
+    write_int(y, msb);
+  }
+  // The inverse of BitReader::read_zeta: h = floor(log2(x + 1) / k) in
+  // unary, then x + 1 - 2^(hk) in a minimal binary code over
+  // [0, 2^((h+1)k) - 2^(hk)).
+  void write_zeta(uint64_t x, unsigned k) {
+    if (k == 1) return write_gamma(x);
+    const uint64_t y = x + 1;
+    const unsigned msb = 63 - __builtin_clzll(y);
+    const unsigned h = msb / k;
+    if ((h + 1) * k > 63) fail(HB_E_INVALID_ARG, "BVGraph writer: zeta code out of range");
+    write_unary(h);
+    const uint64_t left = uint64_t(1) << (h * k);
+    const unsigned s = (h + 1) * k;
+    if (y < 2 * left) write_int(y - left, s - 1);
+    else write_int(y, s);
+  }
+  uint64_t bits() const { return nbits_; }
+  // The bytes, zero-padded to a whole number of 32-bit words as the Java
+  // writer leaves its files.
+  std::vector<uint8_t> take() {
+    if (nbits_ & 7) bytes_.push_back(uint8_t(cur_ << (8 - (nbits_ & 7))));
+    while (bytes_.size() % 4) bytes_.push_back(0);
+    cur_ = 0;
+    return std::move(bytes_);
+  }
+
+ private:
+  void put(uint64_t b) {
+    cur_ = uint8_t((cur_ << 1) | b);
+    if ((++nbits_ & 7) == 0) {
+      bytes_.push_back(cur_);
+      cur_ = 0;
+    }
+  }
+  std::vector<uint8_t> bytes_;
+  uint8_t cur_ = 0;
+  uint64_t nbits_ = 0;
+};
+
+// Counts the bits a code would take, for the reference choice.
+class BitCounter {
+ public:
+  void write_int(uint64_t, unsigned len) { n_ += len; }
+  void write_unary(uint64_t x) { n_ += x + 1; }
+  void write_gamma(uint64_t x) { n_ += 2 * (63 - __builtin_clzll(x + 1)) + 1; }
+  void write_delta(uint64_t x) {
+    const unsigned msb = 63 - __builtin_clzll(x + 1);
+    write_gamma(msb);
+    n_ += msb;
+  }
+  void write_zeta(uint64_t x, unsigned k) {
+    if (k == 1) return write_gamma(x);
+    const uint64_t y = x + 1;
+    const unsigned h = (63 - __builtin_clzll(y)) / k;
+    const uint64_t left = uint64_t(1) << (h * k);
+    n_ += h + 1 + ((h + 1) * k - (y < 2 * left ? 1 : 0));
+  }
+  uint64_t bits() const { return n_; }
+
+ private:
+  uint64_t n_ = 0;
+};
+
+template <class W>
+void write_code(W &w, Code c, unsigned zk, uint64_t x) {
+  switch (c) {
+    case GAMMA: w.write_gamma(x); break;
+    case DELTA: w.write_delta(x); break;
+    case UNARY: w.write_unary(x); break;
+    case ZETA: w.write_zeta(x, zk); break;
+  }
+}
+
+uint64_t int2nat(int64_t v) { return v >= 0 ? uint64_t(v) << 1 : (uint64_t(-(v + 1)) << 1) | 1; }
+
+// Everything of node x's record after the outdegree, given its list and
+// the reference r (0: none) with the referenced node's list.
+template <class W>
+void write_tail(W &w, const BvProps &p, uint64_t x, const uint32_t *list, uint64_t d, uint64_t r,
+                const uint32_t *rl, uint64_t rd, std::vector<uint32_t> &extra) {
+  if (p.window > 0) write_code(w, p.references, p.zeta_k, r);
+  extra.clear();
+  if (r > 0) {
+    // runs over the referenced list of "copied" / "not copied", from a
+    // copy run (possibly empty); the last run is implied by the parity
+    std::vector<uint64_t> runs;
+    uint64_t i = 0, j = 0, run = 0;
+    bool copying = true;
+    for (; i < rd; i++) {
+      while (j < d && list[j] < rl[i]) extra.push_back(list[j++]);
+      const bool in = j < d && list[j] == rl[i];
+      if (in) j++;
+      if (in != copying) {
+        runs.push_back(run);
+        run = 0;
+        copying = in;
+      }
+      run++;
+    }
+    while (j < d) extra.push_back(list[j++]);
+    write_code(w, p.block_count, p.zeta_k, runs.size());
+    for (size_t b = 0; b < runs.size(); b++) write_code(w, p.blocks, p.zeta_k, runs[b] - (b ? 1 : 0));
+  } else {
+    extra.assign(list, list + d);
+  }
+  if (extra.empty()) return;
+  std::vector<uint32_t> resid;
+  if (p.min_interval > 0) {
+    std::vector<std::pair<uint64_t, uint64_t>> iv;
+    for (size_t a = 0; a < extra.size();) {
+      size_t b = a + 1;
+      while (b < extra.size() && extra[b] == extra[b - 1] + 1) b++;
+      if (b - a >= p.min_interval) iv.push_back({extra[a], b - a});
+      else resid.insert(resid.end(), extra.begin() + a, extra.begin() + b);
+      a = b;
+    }
+    w.write_gamma(iv.size());
+    uint64_t prev = 0;
+    for (size_t k = 0; k < iv.size(); k++) {
+      if (k == 0) w.write_gamma(int2nat(int64_t(iv[k].first) - int64_t(x)));
+      else w.write_gamma(iv[k].first - prev - 1);
+      w.write_gamma(iv[k].second - p.min_interval);
+      prev = iv[k].first + iv[k].second;
+    }
+  } else {
+    resid = extra;
+  }
+  uint64_t prev = 0;
+  for (size_t k = 0; k < resid.size(); k++) {
+    if (k == 0) write_code(w, p.residuals, p.zeta_k, int2nat(int64_t(resid[k]) - int64_t(x)));
+    else write_code(w, p.residuals, p.zeta_k, resid[k] - prev - 1);
+    prev = resid[k];
+  }
+}
+
+const char *code_name(Code c) {
+  switch (c) {
+    case GAMMA: return "GAMMA";
+    case DELTA: return "DELTA";
+    case UNARY: return "UNARY";
+    case ZETA: return "ZETA";
+  }
+  return "";
+}
+
+void write_file(const std::string &path, const std::vector<uint8_t> &bytes) {
+  std::ofstream f(path, std::ios::binary | std::ios::trunc);
+  if (!f) fail(HB_E_INVALID_ARG, "cannot create " + path);
+  f.write(reinterpret_cast<const char *>(bytes.data()), std::streamsize(bytes.size()));
+  if (!f) fail(HB_E_INVALID_ARG, "cannot write " + path);
+}
+
+void store_bvgraph(const hb_csr &g, const std::string &basename, const BvProps &p,
+                   uint64_t maxref) {
+  const uint64_t n = g.num_nodes;
+  BitWriter gw, ow;
+  std::vector<uint64_t> refcount(n, 0);
+  std::vector<uint32_t> extra;
+  uint64_t last = 0;
+  for (uint64_t x = 0; x < n; x++) {
+    const uint64_t pos = gw.bits();
+    write_code(ow, p.offsets, p.zeta_k, pos - last);  // gaps between record starts
+    last = pos;
+    const uint32_t *list = g.successors + g.offsets[x];
+    const uint64_t d = g.offsets[x + 1] - g.offsets[x];
+    for (uint64_t k = 1; k < d; k++)
+      if (list[k] <= list[k - 1]) fail(HB_E_INVALID_ARG, "BVGraph writer: rows must be strictly increasing");
+    write_code(gw, p.outdegrees, p.zeta_k, d);
+    if (d == 0) continue;
+    uint64_t best_r = 0, best_bits = UINT64_MAX;
+    for (uint64_t r = 0; r <= std::min<uint64_t>(p.window, x); r++) {
+      const uint64_t y = x - r;
+      const uint64_t rd = r ? g.offsets[y + 1] - g.offsets[y] : 0;
+      if (r > 0 && (rd == 0 || refcount[y] >= maxref)) continue;
+      BitCounter c;
+      write_tail(c, p, x, list, d, r, r ? g.successors + g.offsets[y] : nullptr, rd, extra);
+      if (c.bits() < best_bits) {
+        best_bits = c.bits();
+        best_r = r;
+      }
+    }
+    const uint64_t y = x - best_r;
+    write_tail(gw, p, x, list, d, best_r, best_r ? g.successors + g.offsets[y] : nullptr,
+               best_r ? g.offsets[y + 1] - g.offsets[y] : 0, extra);
+    refcount[x] = best_r ? refcount[y] + 1 : 0;
+  }
+  write_code(ow, p.offsets, p.zeta_k, gw.bits() - last);
+  std::ostringstream props;
+  props << "#BVGraph properties\n"
+        << "graphclass=it.unimi.dsi.webgraph.BVGraph\n"
+        << "version=0\n"
+        << "endianness=big\n"
+        << "nodes=" << n << "\n"
+        << "arcs=" << g.num_arcs << "\n"
+        << "windowsize=" << p.window << "\n"
+        << "maxrefcount=" << maxref << "\n"
+        << "minintervallength=" << p.min_interval << "\n"
+        << "zetak=" << p.zeta_k << "\n";
+  std::string flags;
+  const std::pair<const char *, Code> comps[] = {
+      {"OUTDEGREES", p.outdegrees}, {"REFERENCES", p.references}, {"BLOCKS", p.blocks},
+      {"BLOCK_COUNT", p.block_count}, {"RESIDUALS", p.residuals}, {"OFFSETS", p.offsets}};
+  const Code defaults[] = {GAMMA, UNARY, GAMMA, GAMMA, ZETA, GAMMA};
+  for (int i = 0; i < 6; i++) {
+    if (comps[i].second == defaults[i]) continue;
+    if (!flags.empty()) flags += "|";
+    flags += std::string(comps[i].first) + "_" + code_name(comps[i].second);
+  }
+  props << "compressionflags=" << flags << "\n";
+  const std::string pstr = props.str();
+  write_file(basename + ".graph", gw.take());
+  write_file(basename + ".offsets", ow.take());
+  write_file(basename + ".properties", std::vector<uint8_t>(pstr.begin(), pstr.end()));
+}
+
 template <class T>
 T *copy_out(const std::vector<T> &v) {
   T *p = static_cast<T *>(std::malloc(std::max<size_t>(1, v.size()) * sizeof(T)));
@@ -469,6 +713,54 @@ hb_status hb_bvgraph_load_threads(const char *basename, unsigned threads, hb_gra
     else
       decode_bvgraph(base, off, succ, nullptr);
     fill_graph(out, off.size() - 1, std::move(off), std::move(succ));
+  });
+}
+
+void hb_bvgraph_params_init(hb_bvgraph_params *p) {
+  if (!p) return;
+  p->window_size = 7;
+  p->max_ref_count = 3;
+  p->min_interval_length = 4;
+  p->zeta_k = 3;
+  p->outdegrees = HB_CODE_GAMMA;
+  p->references = HB_CODE_UNARY;
+  p->blocks = HB_CODE_GAMMA;
+  p->block_count = HB_CODE_GAMMA;
+  p->residuals = HB_CODE_ZETA;
+  p->offsets = HB_CODE_GAMMA;
+}
+
+hb_status hb_bvgraph_store(const hb_csr *g, const char *basename, const hb_bvgraph_params *params) {
+  return guard([&] {
+    if (!g || !basename || !params) fail(HB_E_INVALID_ARG, "null argument");
+    if (!g->offsets || (g->num_arcs && !g->successors)) fail(HB_E_INVALID_ARG, "null graph arrays");
+    const uint64_t n = g->num_nodes;
+    if (n > 0xffffffffull) fail(HB_E_INVALID_ARG, "more than 2^32 - 1 nodes");
+    if (g->offsets[0] != 0 || g->offsets[n] != g->num_arcs)
+      fail(HB_E_INVALID_ARG, "offsets must run from 0 to num_arcs");
+    for (uint64_t v = 0; v < n; v++)
+      if (g->offsets[v + 1] < g->offsets[v]) fail(HB_E_INVALID_ARG, "offsets are not monotone");
+    for (uint64_t k = 0; k < g->num_arcs; k++)
+      if (g->successors[k] >= n) fail(HB_E_INVALID_ARG, "successor out of range");
+    const uint32_t codes[6] = {params->outdegrees, params->references, params->blocks,
+                               params->block_count, params->residuals, params->offsets};
+    for (uint32_t c : codes)
+      if (c > HB_CODE_ZETA) fail(HB_E_INVALID_ARG, "unknown code");
+    if (params->zeta_k < 1 || params->zeta_k > 16) fail(HB_E_INVALID_ARG, "zeta_k must be 1..16");
+    if (params->window_size > 1u << 20) fail(HB_E_INVALID_ARG, "window size too large");
+    BvProps p;
+    p.nodes = n;
+    p.arcs = g->num_arcs;
+    p.window = params->window_size;
+    p.min_interval = params->min_interval_length;
+    p.zeta_k = params->zeta_k;
+    p.outdegrees = Code(params->outdegrees);
+    p.references = Code(params->references);
+    p.blocks = Code(params->blocks);
+    p.block_count = Code(params->block_count);
+    p.residuals = Code(params->residuals);
+    p.offsets = Code(params->offsets);
+    store_bvgraph(*g, basename, p, params->max_ref_count);
   });
 }
 

@@ -52,6 +52,15 @@ class _Graph(ctypes.Structure):
                 ("successors", ctypes.POINTER(ctypes.c_uint32))]
 
 
+class _BvParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint32) for k in (
+        "window_size", "max_ref_count", "min_interval_length", "zeta_k", "outdegrees",
+        "references", "blocks", "block_count", "residuals", "offsets")]
+
+
+CODE_GAMMA, CODE_DELTA, CODE_UNARY, CODE_ZETA = 0, 1, 2, 3
+
+
 class _Config(ctypes.Structure):
     _fields_ = [("log2m", ctypes.c_uint32), ("reg_bits", ctypes.c_uint32), ("hash", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("num_elements", ctypes.c_uint64),
@@ -95,6 +104,8 @@ _SIGNATURES = [
     ("hb_bvgraph_decoder_offsets", _I, [ctypes.c_char_p, _P, _U64]),
     ("hb_graph_transpose", _I, [_P, _P]),
     ("hb_bvgraph_load_threads", _I, [_P, _U32, _P]),
+    ("hb_bvgraph_params_init", None, [_P]),
+    ("hb_bvgraph_store", _I, [_P, ctypes.c_char_p, _P]),
     ("hb_graph_transpose_device", _I, [_P, _I, _P]),
     ("hb_graph_generate_uniform", _I, [_U64, _U32, _U64, _P]),
     ("hb_graph_generate_web", _I, [_U64, _D, _U32, _U32, _U64, _D, _U64, _P]),
@@ -242,6 +253,20 @@ class Graph:
         else:
             _check(lib().hb_bvgraph_load_threads(basename.encode(), int(threads), ctypes.byref(g)))
         return cls._from_lib(g)
+
+    def store_bvgraph(self, basename: str, **params) -> None:
+        """Writes the graph as a BVGraph (<basename>.graph, .offsets,
+        .properties); params override the compressor defaults (window_size,
+        max_ref_count, min_interval_length, zeta_k and the CODE_* of
+        outdegrees, references, blocks, block_count, residuals, offsets)."""
+        p = _BvParams()
+        lib().hb_bvgraph_params_init(ctypes.byref(p))
+        for k, v in params.items():
+            if k not in dict(_BvParams._fields_):
+                raise TypeError(f"unknown BVGraph parameter {k}")
+            setattr(p, k, int(v))
+        c = self.csr()
+        _check(lib().hb_bvgraph_store(ctypes.byref(c), basename.encode(), ctypes.byref(p)))
 
     @classmethod
     def uniform(cls, n: int, k: int, seed: int) -> "Graph":
