@@ -193,10 +193,15 @@ def bench_batch(hbm, args):
                    for h in instances for s in h.iteration_stats())
 
     batch = hbm.HyperBallBatch(builders)
-    chosen, table = batch.autotune()
-    print("# batch autotune: " + ", ".join(f"{'none' if t == 2**32 - 1 else t}: {ms:.2f}"
-                                           for t, ms in table) + f" -> {chosen}",
-          file=sys.stderr, flush=True)
+    if args.heavy_threshold is None:
+        chosen, table = batch.autotune()
+        print("# batch autotune: " + ", ".join(f"{'none' if t == 2**32 - 1 else t}: {ms:.2f}"
+                                               for t, ms in table) + f" -> {chosen}",
+              file=sys.stderr, flush=True)
+    else:
+        chosen = args.heavy_threshold
+        for h in batch.instances:
+            h.set_tuning(chosen)
     for _ in range(args.warmup):
         batch.run_until_done()
     t0 = time.perf_counter()
@@ -212,7 +217,10 @@ def bench_batch(hbm, args):
 
     singles = [b.build() for b in builders]
     for h in singles:
-        h.autotune(reps=1)
+        if args.heavy_threshold is None:
+            h.autotune(reps=1)
+        else:
+            h.set_tuning(args.heavy_threshold)
     for _ in range(args.warmup):
         for h in singles:
             h.run_until_done()
@@ -466,7 +474,7 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the uniform n=1e5 leg")
     ap.add_argument("--only-small", action="store_true", help="only the uniform n=1e5 leg")
     ap.add_argument("--heavy-threshold", type=int, default=None,
-                    help="fixed heavy-row threshold instead of the autotuner (for profiling)")
+                    help="fixed heavy-row threshold instead of the autotuners (for profiling)")
     ap.add_argument("--precision", action="store_true",
                     help="only the accumulator precision study (log2m 12 on the web graph)")
     ap.add_argument("--large", action="store_true",
