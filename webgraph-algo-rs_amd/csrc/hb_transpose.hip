@@ -21,11 +21,13 @@
 namespace hb {
 namespace {
 
+// In-degrees in 64 bits: with 2^32 arcs or more one node may have 2^32
+// predecessors.
 __global__ __launch_bounds__(kBlock) void indegree_kernel(const uint32_t *succ, uint64_t a,
-                                                          uint32_t *cnt) {
+                                                          unsigned long long *cnt) {
   for (uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < a;
        k += uint64_t(gridDim.x) * blockDim.x)
-    atomicAdd(cnt + succ[k], 1u);
+    atomicAdd(cnt + succ[k], 1ull);
 }
 
 // Eight lanes per node write the node id over its row.
@@ -62,9 +64,10 @@ hipError_t launch_transpose(const uint64_t *off, const uint32_t *succ, uint64_t 
                             uint64_t *toff, uint32_t *tsucc, hipStream_t s) {
   hipError_t e;
   DeviceScratch cnt, src, keys, tmp;
-  if ((e = cnt.alloc((n + 1) * sizeof(uint32_t))) != hipSuccess) return e;
-  uint32_t *d_cnt = static_cast<uint32_t *>(cnt.p);
-  if ((e = launch_fill_u32(d_cnt, n + 1, 0u, s)) != hipSuccess) return e;
+  if ((e = cnt.alloc((n + 1) * sizeof(uint64_t))) != hipSuccess) return e;
+  unsigned long long *d_cnt = static_cast<unsigned long long *>(cnt.p);
+  if ((e = launch_fill_u32(static_cast<uint32_t *>(cnt.p), 2 * (n + 1), 0u, s)) != hipSuccess)
+    return e;
   if (a) {
     indegree_kernel<<<grid_of(a, kBlock), kBlock, 0, s>>>(succ, a, d_cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
