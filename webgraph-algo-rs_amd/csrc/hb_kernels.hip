@@ -179,14 +179,39 @@ __device__ __forceinline__ void lane_sums(const uint32_t *x, int64_t &s, uint64_
   }
 }
 
-// count() from the lane sums of a whole counter spread over the G lanes of a
-// group; every lane returns the result.  hyper_log_log_logic.rs:169-186.
+// Zero registers among one lane's SPL slabs.
+template <int R, int SPL>
+__device__ __forceinline__ uint32_t lane_zeros(const uint32_t *x) {
+  uint32_t z = 0;
+#pragma unroll
+  for (int k = 0; k < SPL; k++) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < R; b++) o |= x[k * R + b];
+    z += __popc(~o);
+  }
+  return z;
+}
+
+// True when count() takes its linear-counting branch whatever the non-zero
+// registers hold, so that the harmonic sum need not be formed.  The sum is
+// at least zeros - 1: a zero register adds 1, a non-zero one a positive 2^-v
+// or, a 5-bit register of 31, -2^-31 (at most 2^16 of them).  The sum is
+// formed exactly and division rounds monotonically, so the raw estimate
+// alpha_m_m / sum is at most alpha_m_m / (zeros - 1); when that is below
+// 2.5 m the result is lc_table[zeros], as the full computation gives.
+__device__ __forceinline__ bool linear_counting_certain(uint32_t zeros, double alpha_m_m,
+                                                        uint32_t m) {
+  return zeros > 1 && alpha_m_m / double(zeros - 1) < 2.5 * double(m);
+}
+
+// count() from the group's total of zero registers and the lane sums;
+// every lane returns the result.  hyper_log_log_logic.rs:169-186.
 template <int R, int G>
-__device__ __forceinline__ double group_estimate(int64_t s, uint64_t shi, uint64_t slo,
-                                                 uint32_t zeros, double alpha_m_m, uint32_t m,
-                                                 const double *lc_table) {
+__device__ __forceinline__ double estimate_from_sums(int64_t s, uint64_t shi, uint64_t slo,
+                                                     uint32_t zeros, double alpha_m_m,
+                                                     uint32_t m, const double *lc_table) {
   double hm;
-  zeros = group_sum_u32<G>(zeros);
   if constexpr (R == 5) {
     s = int64_t(group_sum_u64<G>(uint64_t(s)));
     hm = double(s) * 0x1p-31;
@@ -203,16 +228,30 @@ __device__ __forceinline__ double group_estimate(int64_t s, uint64_t shi, uint64
   return est;
 }
 
+// count() from the lane sums of a whole counter spread over the G lanes of a
+// group (zeros: this lane's zero registers); every lane returns the result.
+template <int R, int G>
+__device__ __forceinline__ double group_estimate(int64_t s, uint64_t shi, uint64_t slo,
+                                                 uint32_t zeros, double alpha_m_m, uint32_t m,
+                                                 const double *lc_table) {
+  zeros = group_sum_u32<G>(zeros);
+  if (linear_counting_certain(zeros, alpha_m_m, m)) return lc_table[zeros];
+  return estimate_from_sums<R, G>(s, shi, slo, zeros, alpha_m_m, m, lc_table);
+}
+
 // count() of the counter whose slabs are spread over the G lanes of a group;
-// every lane returns the result.
+// every lane returns the result.  A counter with enough zero registers
+// (linear_counting_certain) needs only their number.
 template <int R, int G, int SPL>
 __device__ __forceinline__ double group_count(const uint32_t *x, double alpha_m_m, uint32_t m,
                                               const double *lc_table) {
+  const uint32_t zeros = group_sum_u32<G>(lane_zeros<R, SPL>(x));
+  if (linear_counting_certain(zeros, alpha_m_m, m)) return lc_table[zeros];
   int64_t s = 0;
   uint64_t hi = 0, lo = 0;
-  uint32_t zeros = 0;
-  lane_sums<R, SPL>(x, s, hi, lo, zeros);
-  return group_estimate<R, G>(s, hi, lo, zeros, alpha_m_m, m, lc_table);
+  uint32_t lane_z = 0;
+  lane_sums<R, SPL>(x, s, hi, lo, lane_z);
+  return estimate_from_sums<R, G>(s, hi, lo, zeros, alpha_m_m, m, lc_table);
 }
 
 template <int D>
