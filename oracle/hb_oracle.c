@@ -239,7 +239,8 @@ void or_add(const or_hll *h, uint64_t *c, uint64_t item) {
  * f64` (:178) shifts an i32 literal, so a register of value 31 contributes
  * -2^-31.  Every partial sum is a multiple of 2^-31 below 2^13, so the f64
  * sum is exact and independent of order.
- * For r = 6 (beyond the reference, whose register width is always 5) the
+ * For r = 6 (beyond the reference, whose register width is always 5: its
+ * register_size() is 5 for every usize, weight sums included) the
  * terms are the exact 2^-v, summed as the integer S = sum 2^(63-v) in 128
  * bits and converted as (double)hi * 2^64 + (double)lo, then scaled by
  * 2^-63; the GPU kernels use the same definition. */

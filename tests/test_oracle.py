@@ -18,8 +18,8 @@ from golden_io import assert_array_equal_rel, cnr_golden
 
 def test_register_size_is_five_for_every_n(orc):
     # max(5, ceil(ln((ln n / ln 2) / ln 2))): the parenthesisation keeps the
-    # value at 5 below 2^64 (ln(63/ln 2) = 4.51).
-    for n in [0, 1, 2, 16, 10**4, 10**5, 325557, 10**7, 10**8, 2**32, 2**40, 2**63]:
+    # value at 5 below 2^64 (ln(64/ln 2) = 4.53), weight sums included.
+    for n in [0, 1, 2, 16, 10**4, 10**5, 325557, 10**7, 10**8, 2**32, 2**40, 2**63, 2**64 - 1]:
         assert orc.register_size(n) == 5
 
 
