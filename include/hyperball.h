@@ -70,11 +70,12 @@ typedef struct {
  * tests/test_hyperball.rs:226-227).  Codes: gamma, delta, unary, zeta_k. */
 hb_status hb_bvgraph_load(const char *basename, hb_graph *out);
 /* The same with an explicit host thread count (0: all cores).  With more than
- * one thread and a <basename>.offsets file, node ranges are decoded in
- * parallel from their indexed bit offsets (the random access of
- * BvGraph::with_basename(..).load() through .offsets / Elias-Fano in webgraph
- * 0.2.0); references that leave a range are decoded at their own offsets.
- * One thread: a single sequential pass, no .offsets needed. */
+ * one thread and a node index, node ranges are decoded in parallel from their
+ * indexed bit offsets (the random access of BvGraph::with_basename(..).load()
+ * in webgraph 0.2.0); references that leave a range are decoded at their own
+ * offsets.  The index is <basename>.ef (webgraph's Elias-Fano file) when it
+ * exists, else <basename>.offsets; every range must end where the index
+ * says.  One thread: a single sequential pass, no index needed. */
 hb_status hb_bvgraph_load_threads(const char *basename, unsigned threads, hb_graph *out);
 
 /* Writes a graph in CSR form (rows strictly increasing) as a BVGraph,
@@ -110,6 +111,16 @@ hb_status hb_bvgraph_load_offsets(const char *basename, uint64_t num_nodes, uint
 /* Bit offset at which node v's record starts, as observed by the decoder of
  * hb_bvgraph_load; num_nodes+1 entries. */
 hb_status hb_bvgraph_decoder_offsets(const char *basename, uint64_t *out, uint64_t cap);
+
+/* Decodes an Elias-Fano sequence stored by the sux crate through epserde:
+ * <basename>.ef (the num_nodes+1 bit offsets of the node records, which
+ * BvGraph::with_basename(..).load() reads in webgraph 0.2.0) or
+ * <basename>.dcf (the cumulative out-degrees the reference loads with
+ * DCF::load_mmap, hyperball_impl.rs:1270, main.rs:57).  *count receives the
+ * number of elements; out (cap entries) receives them unless it is NULL.
+ * Errors: HB_E_INVALID_ARG for a file that is not such a sequence, or out
+ * too small. */
+hb_status hb_elias_fano_load(const char *path, uint64_t *out, uint64_t cap, uint64_t *count);
 
 /* Transpose by counting sort; rows of the result are sorted.  Replaces the
  * `-t` transposed BvGraph the reference loads for systolic iterations

@@ -234,3 +234,83 @@ def test_bvgraph_writer_rejects(hbm, tmp_path):
     with pytest.raises(hbm.HyperBallError) as e:
         dup.store_bvgraph(str(tmp_path / "y"))
     assert "strictly increasing" in e.value.message
+
+
+# ---- Elias-Fano node indices and cumulative degrees (webgraph's .ef, .dcf) --
+
+GOLD_IN = os.path.join(os.path.dirname(GOLD), "in-2004")
+
+
+@pytest.mark.parametrize("gold,name", [(GOLD, "cnr-2000"), (GOLD, "cnr-2000-t"),
+                                       (GOLD_IN, "in-2004")])
+def test_elias_fano_index_equals_offsets_file(hbm, gold, name):
+    # The reference's .ef and the Java .offsets are two encodings of the same
+    # node-record bit offsets (in-2004's .graph is not in the fixtures, but
+    # both of its indices are).
+    base = os.path.join(gold, name)
+    n = 0
+    for line in open(base + ".properties"):
+        if line.startswith("nodes="):
+            n = int(line.split("=")[1])
+    ef = hbm.load_elias_fano(base + ".ef")
+    recorded = np.zeros(n + 1, np.uint64)
+    hbm._check(hbm.lib().hb_bvgraph_load_offsets(base.encode(), n, recorded.ctypes.data))
+    assert ef.shape == (n + 1,)
+    assert np.array_equal(ef, recorded)
+    assert ef[0] == 0 and np.all(np.diff(ef.astype(np.int64)) > 0)
+
+
+def test_elias_fano_index_equals_decoder_offsets(hbm):
+    base = os.path.join(GOLD, "cnr-2000")
+    seen = np.zeros(325558, np.uint64)
+    hbm._check(hbm.lib().hb_bvgraph_decoder_offsets(base.encode(), seen.ctypes.data, seen.size))
+    assert np.array_equal(hbm.load_elias_fano(base + ".ef"), seen)
+
+
+def test_dcf_equals_decoded_cumulative_degrees(hbm, cnr):
+    g, _ = cnr
+    dcf = hbm.load_elias_fano(os.path.join(GOLD, "cnr-2000.dcf"))
+    assert np.array_equal(dcf, g.offsets)
+
+
+def test_parallel_decode_through_elias_fano_index(hbm, cnr, tmp_path):
+    # .graph, .properties and .ef only: the parallel decoder takes its node
+    # ranges from the Elias-Fano index
+    import shutil
+    g, _ = cnr
+    for ext in (".graph", ".properties", ".ef"):
+        shutil.copy(os.path.join(GOLD, "cnr-2000" + ext), tmp_path / ("cnr-2000" + ext))
+    base = str(tmp_path / "cnr-2000")
+    for threads in (1, 4, 16):
+        h = hbm.Graph.load_bvgraph(base, threads=threads)
+        assert np.array_equal(h.offsets, g.offsets)
+        assert np.array_equal(h.successors, g.successors)
+    # an index that does not belong to the graph is caught: the transpose's
+    # .ef has the same node count but other record offsets
+    shutil.copy(os.path.join(GOLD, "cnr-2000-t.ef"), tmp_path / "cnr-2000.ef")
+    with pytest.raises(hbm.HyperBallError, match="node index"):
+        hbm.Graph.load_bvgraph(base, threads=4)
+
+
+def test_elias_fano_rejects(hbm, tmp_path):
+    import ctypes
+    src = open(os.path.join(GOLD, "cnr-2000.ef"), "rb").read()
+    cases = {
+        "magic": b"epserdX " + src[8:],
+        "truncated": src[: len(src) // 2],
+        "header": src[:20],
+        "type": src[:37] + src[37:37 + 10].replace(b"sux", b"sox") + src[47:],
+    }
+    for what, data in cases.items():
+        p = tmp_path / (what + ".ef")
+        p.write_bytes(data)
+        with pytest.raises(hbm.HyperBallError, match="Elias-Fano"):
+            hbm.load_elias_fano(str(p))
+    with pytest.raises(hbm.HyperBallError, match="cannot open"):
+        hbm.load_elias_fano(str(tmp_path / "missing.ef"))
+    cnt = ctypes.c_uint64()
+    small = np.zeros(10, np.uint64)
+    with pytest.raises(hbm.HyperBallError, match="too small"):
+        hbm._check(hbm.lib().hb_elias_fano_load(os.path.join(GOLD, "cnr-2000.ef").encode(),
+                                                small.ctypes.data, small.size, ctypes.byref(cnt)))
+    assert cnt.value == 325558

@@ -312,7 +312,121 @@ std::vector<uint64_t> read_bit_offsets(const std::string &basename, const BvProp
   return out;
 }
 
-// Parallel decode through the .offsets index: node ranges balanced by bit
+// ------------------------------------------------------------------------
+// Elias-Fano sequences in the epserde files of the sux crate (<basename>.ef,
+// the bit offsets that webgraph's BvGraph::with_basename(..).load() reads for
+// random access; <basename>.dcf, the cumulative degrees that the reference
+// loads with DCF::load_mmap at hyperball_impl.rs:1270 and main.rs:57).
+//
+// File: the 8-byte magic "epserde ", major and minor version (u16 each), the
+// size of usize (u8), two 64-bit type hashes, the type name (u64 length and
+// bytes), then the fields in declaration order, little-endian.  Scalars
+// follow each other unaligned; a slice is its u64 length, zero padding to
+// an 8-byte file offset, and its words.  EliasFano { n, u, l, low_bits:
+// BitFieldVec { words, bit_width, mask, len }, high_bits }, where the
+// high-bits structure (SelectAdaptConst, or SelectZeroAdaptConst over it)
+// serialises its BitVec { words, len } first; the selection inventories
+// after it are not needed for a full decode.  Element i is
+// ((position of the i-th one in the high bits) - i) << l | low_bits[i].
+// The type hashes are not checked; the structure is (lengths, widths, the
+// high-bits length n + (u >> l) + 1, monotone values up to u).
+struct EfReader {
+  const std::vector<uint8_t> &b;
+  const std::string &path;
+  uint64_t o = 0;
+  [[noreturn]] void bad(const std::string &why) const {
+    fail(HB_E_INVALID_ARG, path + ": not an Elias-Fano epserde file (" + why + ")");
+  }
+  void need(uint64_t k) const {
+    if (o > b.size() || b.size() - o < k) bad("truncated");
+  }
+  uint64_t u64() {
+    need(8);
+    uint64_t v = 0;
+    for (int i = 7; i >= 0; i--) v = (v << 8) | b[o + i];
+    o += 8;
+    return v;
+  }
+  // a slice of words: its offset in the file and its length
+  std::pair<uint64_t, uint64_t> words() {
+    const uint64_t len = u64();
+    o = (o + 7) & ~uint64_t(7);
+    if (len > (b.size() - std::min<uint64_t>(o, b.size())) / 8) bad("slice past the end");
+    const uint64_t at = o;
+    o += len * 8;
+    return {at, len};
+  }
+  uint64_t word(uint64_t at, uint64_t i) const {
+    uint64_t v = 0;
+    for (int k = 7; k >= 0; k--) v = (v << 8) | b[at + i * 8 + k];
+    return v;
+  }
+};
+
+std::vector<uint64_t> read_elias_fano(const std::string &path) {
+  const std::vector<uint8_t> b = read_file(path);
+  EfReader r{b, path};
+  r.need(13);
+  if (std::memcmp(b.data(), "epserde ", 8) != 0) r.bad("no epserde magic");
+  const unsigned major = b[8] | (b[9] << 8);
+  if (major != 1) r.bad("epserde major version " + std::to_string(major));
+  if (b[12] != 8) r.bad("usize of " + std::to_string(b[12]) + " bytes");
+  r.o = 13 + 16;  // the two type hashes
+  const uint64_t name_len = r.u64();
+  r.need(name_len);
+  const std::string name(reinterpret_cast<const char *>(b.data() + r.o), size_t(name_len));
+  r.o += name_len;
+  if (name.rfind("sux::dict::elias_fano::EliasFano<", 0) != 0) r.bad("type " + name);
+  const uint64_t n = r.u64(), u = r.u64(), l = r.u64();
+  if (l > 63) r.bad("low width " + std::to_string(l));
+  const auto low = r.words();
+  const uint64_t bit_width = r.u64(), mask = r.u64(), low_len = r.u64();
+  if (bit_width != l || low_len != n || mask != (l ? (~uint64_t(0) >> (64 - l)) : 0))
+    r.bad("low bits do not match n and l");
+  if (low.second < (n * l + 63) / 64) r.bad("low bits too short");
+  const auto high = r.words();
+  const uint64_t high_len = r.u64();
+  if (high_len != n + (u >> l) + 1 || high.second < (high_len + 63) / 64)
+    r.bad("high bits do not match n, u and l");
+  std::vector<uint64_t> out(n);
+  uint64_t i = 0;
+  for (uint64_t w = 0; w < high.second && i < n; w++) {
+    if (w * 64 >= high_len) break;
+    uint64_t x = r.word(high.first, w);
+    if (high_len - w * 64 < 64) x &= (uint64_t(1) << (high_len - w * 64)) - 1;
+    while (x && i < n) {
+      const uint64_t pos = w * 64 + uint64_t(__builtin_ctzll(x));
+      x &= x - 1;
+      uint64_t lo = 0;
+      if (l) {
+        const uint64_t bit = i * l, wi = bit >> 6, sh = bit & 63;
+        lo = r.word(low.first, wi) >> sh;
+        if (sh + l > 64) lo |= r.word(low.first, wi + 1) << (64 - sh);
+        lo &= mask;
+      }
+      out[i] = ((pos - i) << l) | lo;
+      if (out[i] > u || (i && out[i] < out[i - 1])) r.bad("values not monotone up to u");
+      i++;
+    }
+  }
+  if (i != n) r.bad("fewer ones in the high bits than elements");
+  return out;
+}
+
+// The bit offsets of <basename>.graph's node records, n + 1 of them: from
+// <basename>.ef when it exists (webgraph's own index), else from .offsets.
+std::vector<uint64_t> read_node_index(const std::string &basename, const BvProps &p) {
+  if (std::ifstream(basename + ".ef").good()) {
+    std::vector<uint64_t> bits = read_elias_fano(basename + ".ef");
+    if (bits.size() != p.nodes + 1)
+      fail(HB_E_INVALID_ARG, basename + ".ef holds " + std::to_string(bits.size()) +
+                                 " offsets, the graph has " + std::to_string(p.nodes) + " nodes");
+    return bits;
+  }
+  return read_bit_offsets(basename, p);
+}
+
+// Parallel decode through the node index (.ef or .offsets): node ranges balanced by bit
 // length are decoded by separate threads.  A reference to a node before a
 // thread's range is resolved by decoding that node at its indexed bit offset
 // (recursively through its own references), with a per-thread cache.  Every
@@ -321,11 +435,12 @@ void decode_bvgraph_parallel(const std::string &basename, unsigned threads,
                              std::vector<uint64_t> &offsets, std::vector<uint32_t> &succ) {
   const BvProps p = read_properties(basename);
   const uint64_t n = p.nodes;
-  const std::vector<uint64_t> bits = read_bit_offsets(basename, p);
+  const std::vector<uint64_t> bits = read_node_index(basename, p);
   const BitFile gf(read_file(basename + ".graph"));
-  if (bits[n] > gf.nbits) fail(HB_E_INVALID_ARG, "BVGraph: .offsets points past the .graph stream");
+  if (bits[n] > gf.nbits)
+    fail(HB_E_INVALID_ARG, "BVGraph: the node index points past the .graph stream");
   for (uint64_t x = 0; x < n; x++)
-    if (bits[x + 1] < bits[x]) fail(HB_E_INVALID_ARG, "BVGraph: .offsets are not monotone");
+    if (bits[x + 1] < bits[x]) fail(HB_E_INVALID_ARG, "BVGraph: the node index is not monotone");
   const uint64_t want = std::max<uint64_t>(1, std::min<uint64_t>(n, uint64_t(threads) * 8));
   std::vector<uint64_t> bounds{0};
   for (uint64_t c = 1; c < want; c++) {
@@ -369,14 +484,14 @@ void decode_bvgraph_parallel(const std::string &basename, unsigned threads,
         for (uint64_t x = b; x < e; x++) {
           lo.push_back(ls.size());  // also the end of x - 1's list, for ref()
           if (in.position() != bits[x])
-            fail(HB_E_INVALID_ARG, "BVGraph: .offsets disagree with the .graph stream at node " +
+            fail(HB_E_INVALID_ARG, "BVGraph: the node index disagrees with the .graph stream at node " +
                                        std::to_string(x));
           decode_list(in, p, x, ref, merged, blocks);
           ls.insert(ls.end(), merged.begin(), merged.end());
         }
         lo.push_back(ls.size());
         if (in.position() != bits[e])
-          fail(HB_E_INVALID_ARG, "BVGraph: .offsets disagree with the .graph stream at node " +
+          fail(HB_E_INVALID_ARG, "BVGraph: the node index disagrees with the .graph stream at node " +
                                      std::to_string(e));
       }
     } catch (...) {
@@ -708,7 +823,8 @@ hb_status hb_bvgraph_load_threads(const char *basename, unsigned threads, hb_gra
     std::vector<uint32_t> succ;
     if (threads == 0) threads = host_threads();
     const std::string base(basename);
-    if (threads > 1 && std::ifstream(base + ".offsets").good())
+    if (threads > 1 &&
+        (std::ifstream(base + ".ef").good() || std::ifstream(base + ".offsets").good()))
       decode_bvgraph_parallel(base, threads, off, succ);
     else
       decode_bvgraph(base, off, succ, nullptr);
@@ -787,6 +903,17 @@ hb_status hb_bvgraph_load_offsets(const char *basename, uint64_t num_nodes, uint
       fail(HB_E_INVALID_ARG, "BVGraph: the graph has " + std::to_string(p.nodes) + " nodes");
     const std::vector<uint64_t> bits = read_bit_offsets(basename, p);
     std::memcpy(out, bits.data(), bits.size() * sizeof(uint64_t));
+  });
+}
+
+hb_status hb_elias_fano_load(const char *path, uint64_t *out, uint64_t cap, uint64_t *count) {
+  return guard([&] {
+    if (!path || !count) fail(HB_E_INVALID_ARG, "null argument");
+    const std::vector<uint64_t> v = read_elias_fano(path);
+    *count = v.size();
+    if (!out) return;
+    if (cap < v.size()) fail(HB_E_INVALID_ARG, "output buffer too small");
+    std::memcpy(out, v.data(), v.size() * sizeof(uint64_t));
   });
 }
 

@@ -102,6 +102,7 @@ _SIGNATURES = [
     ("hb_bvgraph_load", _I, [ctypes.c_char_p, _P]),
     ("hb_bvgraph_load_offsets", _I, [ctypes.c_char_p, _U64, _P]),
     ("hb_bvgraph_decoder_offsets", _I, [ctypes.c_char_p, _P, _U64]),
+    ("hb_elias_fano_load", _I, [ctypes.c_char_p, _P, _U64, ctypes.POINTER(ctypes.c_uint64)]),
     ("hb_graph_transpose", _I, [_P, _P]),
     ("hb_bvgraph_load_threads", _I, [_P, _U32, _P]),
     ("hb_bvgraph_params_init", None, [_P]),
@@ -202,6 +203,17 @@ def log_2_num_of_registers(rsd: float) -> int:
 # Graphs
 # ---------------------------------------------------------------------------
 
+def load_elias_fano(path: str) -> np.ndarray:
+    """The elements of an Elias-Fano sequence that the sux crate stored
+    through epserde: a BVGraph's .ef (bit offsets of the node records) or
+    .dcf (cumulative out-degrees)."""
+    cnt = ctypes.c_uint64()
+    _check(lib().hb_elias_fano_load(path.encode(), None, 0, ctypes.byref(cnt)))
+    out = np.empty(cnt.value, dtype=np.uint64)
+    _check(lib().hb_elias_fano_load(path.encode(), _ptr(out), out.size, ctypes.byref(cnt)))
+    return out
+
+
 class Graph:
     """A CSR graph: successors of v are successors[offsets[v]:offsets[v+1]]."""
 
@@ -246,7 +258,8 @@ class Graph:
     @classmethod
     def load_bvgraph(cls, basename: str, threads: Optional[int] = None) -> "Graph":
         """Decodes <basename>.graph (BVGraph v0); threads None = all cores,
-        in parallel through <basename>.offsets when it exists."""
+        in parallel through the node index, <basename>.ef or .offsets, when
+        one exists."""
         g = _Graph()
         if threads is None:
             _check(lib().hb_bvgraph_load(basename.encode(), ctypes.byref(g)))
