@@ -303,6 +303,25 @@ __device__ __forceinline__ void acc_add(void *base, uint32_t mode, uint32_t v, d
   }
 }
 
+// acc_add onto an accumulator known to hold zero (iteration 0: init zeroes
+// them): the same arithmetic on a zero operand, with no read.
+__device__ __forceinline__ void acc_add_to_zero(void *base, uint32_t mode, uint32_t v, double x) {
+  if (mode == ACC_F64) {
+    static_cast<double *>(base)[v] = 0.0 + x;
+  } else if (mode == ACC_F32_KAHAN) {
+    float2 z;
+    z.x = 0.0f;
+    z.y = 0.0f;
+    const float y = float(x) - z.y;
+    const float t = z.x + y;
+    z.y = (t - z.x) - y;
+    z.x = t;
+    static_cast<float2 *>(base)[v] = z;
+  } else {
+    static_cast<float *>(base)[v] = 0.0f + float(x);
+  }
+}
+
 // Wavefront sum of the per-lane accumulators, added by lane 0 to the
 // wavefront's statistics slot, or to *st when `slots` is null.  Every lane of
 // the wavefront must call it.
@@ -466,7 +485,8 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
     if (!DELTA) {
 #pragma unroll
       for (int j = 0; j < K; j++) {
-        f[j] = (s[j] != v) && test_bit(a.curr_mod, s[j]);
+        // seeded counters: init marks every node modified
+        f[j] = (s[j] != v) && (SEEDED || test_bit(a.curr_mod, s[j]));
         mk[j] = ~0u;
       }
       if (SEEDED) {
@@ -502,26 +522,48 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
     if (!cnt) continue;
     lane_sync<G>();
     if (!loaded) {
-      load_part<D>(prev, own);
+      if constexpr (SEEDED) {
+        // the node's own counter holds the one register of its own hash
+        uint32_t oreg = 0, oval = 0;
+        seed_register(a.hash, a.seed, a.log2m, a.sentinel, v, oreg, oval);
+        const uint32_t oslab = oreg >> 5, obit = oreg & 31;
+#pragma unroll
+        for (int q = 0; q < SPL; q++)
+#pragma unroll
+          for (int b = 0; b < R; b++)
+            prev[q * R + b] = oslab == slab0 + q ? ((oval >> b) & 1u) << obit : 0u;
+      } else {
+        load_part<D>(prev, own);
+      }
 #pragma unroll
       for (int k = 0; k < D; k++) cur[k] = prev[k];
       loaded = true;
     }
     nvisit += cnt;
     if (SEEDED) {
-      for (uint32_t k = 0; k < cnt; k++) {
-        const uint32_t w = ids[k];
-        const uint32_t reg = w & 0xffffu, val = w >> 16, slab = reg >> 5, bit = reg & 31;
+      // eight slots per batch, read before use; a batch may read up to seven
+      // slots past cnt, inside the group's 2 * slice_width<G>() words
+      static_assert(slice_width<G>() >= 8, "seeded batches read within the slot");
+      for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
+        uint32_t w8[8];
 #pragma unroll
-        for (int q = 0; q < SPL; q++) {
-          if (slab != slab0 + q) continue;
-          uint32_t cv = 0;
+        for (int u = 0; u < 8; u++) w8[u] = ids[k0 + u];
 #pragma unroll
-          for (int b = 0; b < R; b++) cv |= ((cur[q * R + b] >> bit) & 1u) << b;
-          if (val > cv) {
+        for (int u = 0; u < 8; u++) {
+          if (k0 + u >= cnt) break;
+          const uint32_t w = w8[u];
+          const uint32_t reg = w & 0xffffu, val = w >> 16, slab = reg >> 5, bit = reg & 31;
 #pragma unroll
-            for (int b = 0; b < R; b++)
-              cur[q * R + b] = (cur[q * R + b] & ~(1u << bit)) | (((val >> b) & 1u) << bit);
+          for (int q = 0; q < SPL; q++) {
+            if (slab != slab0 + q) continue;
+            uint32_t cv = 0;
+#pragma unroll
+            for (int b = 0; b < R; b++) cv |= ((cur[q * R + b] >> bit) & 1u) << b;
+            if (val > cv) {
+#pragma unroll
+              for (int b = 0; b < R; b++)
+                cur[q * R + b] = (cur[q * R + b] & ~(1u << bit)) | (((val >> b) & 1u) << bit);
+            }
           }
         }
       }
@@ -589,22 +631,30 @@ __device__ __forceinline__ void node_update(const UnionArgs &a, uint32_t v, int 
                                             bool cm, uint32_t chg_bits, double post,
                                             uint32_t nvisit, uint64_t deg, Acc &acc_out) {
   const bool systolic = a.flags & F_SYSTOLIC;
+  // seeded counters (iteration 0 without weights): every cached count is the
+  // seeding's lc[m - 1] and every accumulator is zero, so neither is read
+  const bool seeded = a.flags & F_SEEDED;
   if (gl == 0) {
     double pre = 0.0;
     if (cm) {
       a.chg_next[v] = chg_bits;
       acc_out.changed += __popc(chg_bits);
-      pre = a.cache[v];
+      pre = seeded ? a.lc_table[a.m - 1] : a.cache[v];
       a.cache[v] = post;
     } else if (!systolic && check) {
-      post = a.cache[v];
+      post = seeded ? a.lc_table[a.m - 1] : a.cache[v];
     }
     if (!systolic && check) acc_out.add_nf(post);
     if (cm) {
       if (systolic) acc_out.add_nf(post - pre);
       if (a.flags & F_CENTRALITY) {
         const double delta = post - pre;
-        if (delta > 0.0) {
+        if (delta > 0.0 && seeded) {
+          if (a.sd) acc_add_to_zero(a.sd, a.acc_mode, v, delta * a.dist);
+          if (a.hc) acc_add_to_zero(a.hc, a.acc_mode, v, delta / a.dist);
+          for (int k = 0; k < a.ndisc; k++)
+            acc_add_to_zero(a.dc[k], a.acc_mode, v, delta * a.disc[k]);
+        } else if (delta > 0.0) {
           if (a.sd) acc_add(a.sd, a.acc_mode, v, delta * a.dist);
           if (a.hc) acc_add(a.hc, a.acc_mode, v, delta / a.dist);
           for (int k = 0; k < a.ndisc; k++) acc_add(a.dc[k], a.acc_mode, v, delta * a.disc[k]);
