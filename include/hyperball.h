@@ -291,6 +291,8 @@ typedef struct {
                                were gathered (chosen per iteration) */
   uint32_t streamed;      /* HB_COUNTERS_STREAM: 1 when the iteration streamed the counters
                              through device memory, 0 when its kernels read them in place */
+  uint32_t window;        /* 1 when the union step ran as the window kernel (hb_set_window) */
+  uint32_t reserved;
 } hb_iteration_stats;
 hb_status hb_num_iterations(hb_instance *hb, uint64_t *out);
 hb_status hb_get_iteration_stats(hb_instance *hb, uint64_t iteration, hb_iteration_stats *out);
@@ -316,6 +318,21 @@ hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree);
  * iteration, when counters have at least 1024 registers and the expected
  * share of changed slabs is below 0.7; 1: always; 2: never. */
 hb_status hb_set_delta_gathering(hb_instance *hb, int mode);
+/* Window kernel, for 64-register counters (log2m 6) in device memory: in an
+ * iteration that checks every node and reads the counters (not the seeded
+ * first one), each workgroup takes blocks of consecutive nodes and first
+ * copies the current counters of the nodes around its block (the tile:
+ * half_width nodes on each side) into LDS, where successors inside it are
+ * read; the others are read from HBM.  Exact, like every kernel choice.
+ * mode 0 (default) and 1: in those iterations; 2: never.  block_nodes and
+ * half_width cap the kernel's own block and tile (0: the kernel's largest);
+ * rows longer than heavy_degree take a workgroup each (0: the larger of
+ * hb_set_tuning's threshold and 8192; UINT32_MAX: none), shorter ones a
+ * wavefront (above 64 successors) or a lane pair inside their block.  For
+ * tests and tuning.  No counterpart in the reference (its union step is
+ * hyperball_impl.rs:1025-1145 on CPU threads). */
+hb_status hb_set_window(hb_instance *hb, int mode, uint32_t block_nodes, uint32_t half_width,
+                        uint32_t heavy_degree);
 /* Streamed counters (HB_COUNTERS_STREAM): per iteration, either stream the
  * counter arrays through device memory tile by tile, or let the union kernel
  * read and write them in place over the host link, as HB_COUNTERS_HOST does.

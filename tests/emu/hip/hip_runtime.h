@@ -55,6 +55,7 @@ struct float2 {
 struct uint2 {
   uint32_t x, y;
 };
+inline uint2 make_uint2(uint32_t x, uint32_t y) { return uint2{x, y}; }
 struct uint4 {
   uint32_t x, y, z, w;
 };

@@ -2,9 +2,11 @@
 // engine, in lockstep (test infrastructure, built by build_emu.py): after
 // every iteration the current registers (reference layout), the modified
 // bits and the iteration mode must be identical; at the end, the sums of
-// distances.  Usage:
+// distances.  Visited arcs and modified counts of every iteration must equal
+// the oracle's too.  Usage:
 //   oracle_check LOG2M REG_BITS N DMAX SEED HEAVY [DELTA_MODE [COUNTER_MEMORY TILE_NODES
 //                [STREAMING_MODE]]]
+// HB_CHECK_WINDOW="MODE BLOCK HALF HEAVY" in the environment sets hb_set_window.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -57,6 +59,15 @@ int main(int argc, char **argv) {
   if (heavy) check(hb_set_tuning(hb, heavy), "hb_set_tuning");
   check(hb_set_delta_gathering(hb, delta_mode), "hb_set_delta_gathering");
   if (counter_memory == HB_COUNTERS_STREAM) check(hb_set_streaming(hb, streaming), "hb_set_streaming");
+  if (const char *w = std::getenv("HB_CHECK_WINDOW")) {
+    unsigned wm = 0, wb = 0, wh = 0, wy = 0;
+    if (std::sscanf(w, "%u %u %u %u", &wm, &wb, &wh, &wy) != 4) {
+      std::fprintf(stderr, "HB_CHECK_WINDOW: MODE BLOCK HALF HEAVY\n");
+      return 2;
+    }
+    check(hb_set_window(hb, int(wm), wb, wh, wy), "hb_set_window");
+    std::printf("window mode %u block %u half %u heavy %u\n", wm, wb, wh, wy);
+  }
   or_hb *ref = or_hb_create(n, gg.num_arcs, gg.offsets, gg.successors, gt.offsets, gt.successors,
                             log2m, r, OR_HASH_JENKINS, seed, 1, 0, nullptr, 0, 1);
   if (!ref) {
@@ -91,12 +102,16 @@ int main(int argc, char **argv) {
     const bool same_c = std::memcmp(c.data(), or_hb_counters(ref, 0), words * 8) == 0;
     const bool same_m = std::memcmp(mod.data(), or_hb_modified_bits(ref), mod.size() * 8) == 0;
     const bool same_mode = mode == or_hb_mode(ref);
-    std::printf("iteration %d: mode %d modified %llu delta %u streamed %u gathered %llu of %llu: %s\n",
-                it, mode, (unsigned long long)st.modified, st.delta_gathering, st.streamed,
-                (unsigned long long)st.gathered_slabs,
+    const bool same_stats =
+        st.visited_arcs == or_hb_visited_arcs(ref) && st.modified == or_hb_modified(ref);
+    std::printf("iteration %d: mode %d modified %llu visited %llu (oracle %llu) delta %u "
+                "streamed %u window %u gathered %llu of %llu: %s\n",
+                it, mode, (unsigned long long)st.modified, (unsigned long long)st.visited_arcs,
+                (unsigned long long)or_hb_visited_arcs(ref), st.delta_gathering, st.streamed,
+                st.window, (unsigned long long)st.gathered_slabs,
                 (unsigned long long)(st.visited_arcs * (p.num_registers / 32)),
-                same_c && same_m && same_mode ? "identical" : "DIFFERENT");
-    bad += !(same_c && same_m && same_mode);
+                same_c && same_m && same_mode && same_stats ? "identical" : "DIFFERENT");
+    bad += !(same_c && same_m && same_mode && same_stats);
     if (or_hb_modified(ref) == 0 || bad) break;
   }
   std::vector<double> sd(n);

@@ -66,6 +66,25 @@ def test_emulated_engine_matches_oracle(emu_exes, log2m, r, n, dmax, seed, heavy
     assert "DIFFERENT" not in res.stdout
 
 
+@pytest.mark.parametrize("r,n,dmax,seed,window", [
+    (5, 3000, 300, 9, "0 0 0 0"),        # one block, the whole graph in the tile
+    (6, 2000, 600, 11, "0 64 40 100"),   # 64-node blocks, most successors outside the tile,
+                                         # rows above 100 on the whole workgroup
+    (5, 2500, 400, 4, "0 128 0 4294967295"),  # the largest tile around 128-node blocks, every
+                                              # long row on a wavefront
+])
+def test_emulated_window_kernel_matches_oracle(emu_exes, r, n, dmax, seed, window):
+    """The window kernel (log2m 6, iterations that check every node; delta
+    gathering off so that they all take it) against the oracle every
+    iteration, visited arcs included, under AddressSanitizer."""
+    args = [emu_exes["oracle_check"], "6", str(r), str(n), str(dmax), str(seed), "64", "2"]
+    env = dict(os.environ, HB_CHECK_WINDOW=window)
+    res = subprocess.run(args, capture_output=True, text=True, timeout=900, env=env)
+    assert res.returncode == 0 and res.stdout.strip().endswith("OK"), res.stdout[-3000:] + res.stderr[-6000:]
+    assert "DIFFERENT" not in res.stdout
+    assert res.stdout.count(" window 1 ") >= 5
+
+
 def test_emulated_count_high_registers_match_oracle(emu_exes):
     """count() on the device path (count_all_kernel, every shape) equals the
     oracle's or_count on crafted counters whose registers sit in the high

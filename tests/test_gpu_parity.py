@@ -28,7 +28,7 @@ def _tiny_graphs(hbm):
 
 
 def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_000, heavy=None,
-                  delta=None, counter_memory=None, tile_nodes=0, streaming=None):
+                  delta=None, counter_memory=None, tile_nodes=0, streaming=None, window=None):
     b = hbm.HyperBallBuilder.with_hyper_log_log(g, gt, log2m).register_bits(reg_bits)
     if jenkins is not None:
         b = b.jenkins(jenkins)
@@ -43,6 +43,8 @@ def _run_lockstep(hbm, orc, g, gt, log2m, reg_bits=0, jenkins=None, max_iter=10_
         hb.set_delta_gathering(delta)
     if streaming is not None:
         hb.set_streaming(streaming)
+    if window is not None:
+        hb.set_window(*window)
     hll = orc.Hll(log2m, reg_bits, g.num_nodes, orc.HASH_JENKINS if jenkins is not None else orc.HASH_SIPHASH13,
                   jenkins or 0)
     ref = orc.Jacobi(hll, g.offsets, g.successors)
@@ -181,6 +183,67 @@ def test_delta_gathering_bit_exact(hbm, orc, delta, log2m, reg_bits):
         assert gathered == total
     else:
         assert gathered < total
+
+
+@pytest.mark.parametrize("window", [(0, 64, 40, 100), (0, 256, 200, 64), (0, 128, 0, 2**32 - 1),
+                                    (0, 0, 0, 0), (2, 0, 0, 0)])
+@pytest.mark.parametrize("reg_bits", [5, 6])
+def test_window_kernel_bit_exact(hbm, orc, window, reg_bits):
+    """The window kernel (64-register counters, iterations that check every
+    node): every iteration bit-exact against the Jacobi oracle, with blocks
+    and tiles small enough that most successors lie outside the tile, rows
+    taking a wavefront or a workgroup, and switched off (mode 2).  Without a
+    transpose every iteration checks every node, so every one but the seeded
+    first runs the window kernel."""
+    g = hbm.Graph.web(20000, 2.1, 1, 2000, 300, 0.7, 17)
+    it, stats = _run_lockstep(hbm, orc, g, None, 6, reg_bits, jenkins=3, window=window)
+    assert it > 5
+    assert [s["window"] for s in stats] == [int(i > 0 and window[0] != 2) for i in range(it)]
+
+
+@pytest.mark.parametrize("reg_bits", [5, 6])
+@pytest.mark.parametrize("acc", ["ACC_F64", "ACC_F32_KAHAN", "ACC_F32"])
+def test_window_kernel_engine_outputs(hbm, orc, reg_bits, acc):
+    """Window iterations against the oracle's restatement of the reference
+    engine, with a transpose (dense iterations, then systolic ones on the
+    union kernel): registers, modified bits and the visited arcs the kernel
+    counts from in-degrees, every iteration; sums of distances and harmonic
+    centralities bit-exact in f64 and within 1e-6 relative with f32
+    accumulators (Kahan-compensated or not)."""
+    g = hbm.Graph.web(30000, 2.1, 1, 3000, 400, 0.7, 21)
+    gt = g.transpose()
+    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 6).register_bits(reg_bits).jenkins(11)
+          .sum_of_distances(True).sum_of_inverse_distances(True)
+          .accumulator_precision(getattr(hbm, acc)).build())
+    hb.set_window(0, 512, 300, 200)
+    ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=6,
+                              reg_bits=reg_bits, hash=orc.HASH_JENKINS, seed=11,
+                              sum_of_distances=True, sum_of_inverse_distances=True)
+    hb.init()
+    ref.init()
+    windows = 0
+    for it in range(1000):
+        hb.iterate()
+        ref.iterate()
+        s = hb.iteration_stats()[-1]
+        windows += s["window"]
+        assert np.array_equal(hb.counters(0), ref.counters(0)), f"registers differ at {it}"
+        assert np.array_equal(hb.modified_bits(), ref.modified_bits()), f"modified bits differ at {it}"
+        assert s["visited_arcs"] == ref.visited_arcs, f"visited arcs differ at {it}"
+        assert s["modified"] == ref.modified
+        if ref.modified == 0:
+            break
+    assert windows >= 3
+    if acc == "ACC_F64":
+        assert np.array_equal(hb.sum_of_distances(), ref.sum_of_distances())
+        assert np.array_equal(hb.harmonic_centralities(), ref.harmonic_centralities())
+    else:
+        np.testing.assert_allclose(hb.sum_of_distances(), ref.sum_of_distances(), rtol=1e-6, atol=0)
+        np.testing.assert_allclose(hb.harmonic_centralities(), ref.harmonic_centralities(),
+                                   rtol=1e-6, atol=0)
+    np.testing.assert_allclose(hb.neighborhood_function(), ref.neighborhood_function(), rtol=1e-12,
+                               atol=0)
+    hb.close()
 
 
 @pytest.mark.parametrize("log2m,reg_bits", [(6, 5), (8, 6), (10, 6), (12, 5)])

@@ -134,6 +134,21 @@ struct Instance {
   uint64_t last_changed = 0;  // change-mask bits set by the last iteration
   bool delta = false;         // the last planned iteration gathers deltas
   int delta_mode = 0;         // 0: by the rule above, 1: always, 2: never
+  // Window kernel (launch_window_union): the union step of iterations that
+  // check every node and read the counters, for 64-register counters in
+  // device memory.  indeg[s] counts the arcs into s from other nodes (the
+  // kernel counts visited arcs from it).  Rows longer than kWindowHeavy (or
+  // the heavy-row threshold, if larger) take a workgroup each; shorter ones a
+  // wavefront or a lane group inside their block.  Successors outside the
+  // tile are read untested while at least kUntestedShare of the nodes changed
+  // in the last iteration.
+  static constexpr uint64_t kWindowHeavy = 8192;
+  static constexpr double kUntestedShare = 0.9;
+  DevBuf<uint32_t> indeg;
+  int window_mode = 0;  // 0: by the rule above, 1: the same, 2: never
+  uint32_t win_block = 0, win_half = 0;
+  uint64_t win_heavy = 0;  // 0: max(heavy_degree, kWindowHeavy)
+  bool window_now = false;  // the iteration under way runs the window kernel
   DevBuf<double> cache, lc;
   DevBuf<uint8_t> sd, hc;
   std::vector<std::unique_ptr<DevBuf<uint8_t>>> dc;
@@ -422,6 +437,7 @@ struct Instance {
     hs.gathered_slabs = st.gathered;
     hs.delta_gathering = delta;
     hs.streamed = streamed_now;
+    hs.window = window_now;
     last_changed = st.changed;
     hs.union_ms = union_ms;
     hs.iteration_ms = iteration_ms;
@@ -585,6 +601,26 @@ struct Instance {
     return visited >= streamed;
   }
 
+  // Turns a planned dense iteration into a window-kernel launch: untested
+  // reads outside the tile while most nodes changed, and the kernel's own
+  // heavy-row threshold (a prefix of heavy_nodes, sorted by decreasing degree).
+  bool plan_window(Plan &p, uint64_t modified_prev) {
+    const uint32_t f = p.ua.flags;
+    if (window_mode == 2 || !indeg.p || p.list_mode || !(f & F_CHECK_ALL) ||
+        (f & (F_LIST | F_SEEDED | F_DELTA | F_SYSTOLIC)))
+      return false;
+    if (double(modified_prev) >= kUntestedShare * double(n)) p.ua.flags |= F_UNTESTED;
+    const uint64_t h = win_heavy ? win_heavy : std::max(heavy_degree, kWindowHeavy);
+    uint64_t k = 0;
+    while (k < heavy_degs.size() && heavy_degs[k] > h) k++;
+    p.ua.heavy_degree = k ? h : UINT64_MAX;
+    p.ua.heavy_count = k;
+    p.ua.indeg = indeg.p;
+    p.ua.win_block = win_block;
+    p.ua.win_half = win_half;
+    return true;
+  }
+
   void iterate() {
     streamed_now = stream_this_iteration();
     if (streamed_now) {
@@ -592,7 +628,9 @@ struct Instance {
       return;
     }
     use_device();
+    const uint64_t modified_prev = modified;
     Plan p = plan_iteration();
+    window_now = plan_window(p, modified_prev);
     if (timing) HB_HIP(hipEventRecord(ev[0], stream));
     HB_HIP(launch_prepare_iteration(p.ua.next_mod, systolic ? p.ua.next_must : nullptr,
                                     bitmap_words, stats.p, work.p, stat_slots.p, stream));
@@ -605,7 +643,10 @@ struct Instance {
     if (p.list_mode)
       HB_HIP(launch_compact(p.ua.must, p.ua.curr_mod, n, bsum.p, list.p, list_len.p, stream));
     if (timing) HB_HIP(hipEventRecord(ev[1], stream));
-    HB_HIP(launch_union(int(r), int(log2m), p.ua, p.grid, stream));
+    if (window_now)
+      HB_HIP(launch_window_union(int(r), int(log2m), p.ua, stream));
+    else
+      HB_HIP(launch_union(int(r), int(log2m), p.ua, p.grid, stream));
     if (timing) HB_HIP(hipEventRecord(ev[2], stream));
     HB_HIP(launch_reduce_stats(stats.p, stat_slots.p, h_stats, stream));  // h_stats: pinned
     if (timing) HB_HIP(hipEventRecord(ev[3], stream));
@@ -1087,6 +1128,12 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
     I.stat_slots.alloc(size_t(kStatSlots) * kStatSlotWords);
     I.work.alloc(kWorkCounters * kWorkStride);
     I.counter_memory = cfg->counter_memory;
+    if (window_union_supported(int(r), int(cfg->log2m)) &&
+        I.counter_memory == HB_COUNTERS_DEVICE && n > 0) {
+      I.indeg.alloc(n);
+      HB_HIP(launch_fill_u32(I.indeg.p, n, 0u, s));
+      HB_HIP(launch_indegree(I.off.p, I.succ.p, n, I.indeg.p, s));
+    }
     if (I.counter_memory == HB_COUNTERS_STREAM) {
       const uint64_t node_bytes = uint64_t(I.cstride) * 4;
       uint64_t t = cfg->stream_tile_nodes ? cfg->stream_tile_nodes
@@ -1452,6 +1499,20 @@ hb_status hb_set_delta_gathering(hb_instance *hb, int mode) {
   return guard([&] {
     if (mode < 0 || mode > 2) fail(HB_E_INVALID_ARG, "delta gathering mode must be 0, 1 or 2");
     checked(hb)->delta_mode = mode;
+  });
+}
+
+hb_status hb_set_window(hb_instance *hb, int mode, uint32_t block_nodes, uint32_t half_width,
+                        uint32_t heavy_degree) {
+  return guard([&] {
+    if (mode < 0 || mode > 2) fail(HB_E_INVALID_ARG, "window mode must be 0, 1 or 2");
+    Instance &I = *checked(hb);
+    I.window_mode = mode;
+    I.win_block = block_nodes;
+    I.win_half = half_width;
+    I.win_heavy = heavy_degree == 0 ? 0
+                  : heavy_degree == UINT32_MAX ? UINT64_MAX
+                                               : std::max<uint64_t>(heavy_degree, Instance::kHeavyFloor);
   });
 }
 

@@ -303,6 +303,51 @@ __device__ __forceinline__ void acc_add(void *base, uint32_t mode, uint32_t v, d
   }
 }
 
+// acc_add in three steps, for callers that read the accumulator early: the
+// raw 8 bytes of node v's accumulator (4 in the f32 mode, in .x), the updated
+// raw value, and its store.  acc_store(acc_apply(acc_load(...), x)) is
+// acc_add.
+__device__ __forceinline__ uint2 acc_load(const void *base, uint32_t mode, uint32_t v) {
+  uint2 r;
+  if (mode == ACC_F32) {
+    r.x = static_cast<const uint32_t *>(base)[v];
+    r.y = 0;
+  } else {
+    r = static_cast<const uint2 *>(base)[v];
+  }
+  return r;
+}
+
+__device__ __forceinline__ uint2 acc_apply(uint2 raw, uint32_t mode, double x) {
+  uint2 r;
+  if (mode == ACC_F64) {
+    const double d =
+        __builtin_bit_cast(double, uint64_t(raw.y) << 32 | uint64_t(raw.x)) + x;
+    const uint64_t b = __builtin_bit_cast(uint64_t, d);
+    r.x = uint32_t(b);
+    r.y = uint32_t(b >> 32);
+  } else if (mode == ACC_F32_KAHAN) {
+    float vx = __builtin_bit_cast(float, raw.x), vy = __builtin_bit_cast(float, raw.y);
+    const float y = float(x) - vy;
+    const float t = vx + y;
+    vy = (t - vx) - y;
+    vx = t;
+    r.x = __builtin_bit_cast(uint32_t, vx);
+    r.y = __builtin_bit_cast(uint32_t, vy);
+  } else {
+    r.x = __builtin_bit_cast(uint32_t, __builtin_bit_cast(float, raw.x) + float(x));
+    r.y = 0;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void acc_store(void *base, uint32_t mode, uint32_t v, uint2 raw) {
+  if (mode == ACC_F32)
+    static_cast<uint32_t *>(base)[v] = raw.x;
+  else
+    static_cast<uint2 *>(base)[v] = raw;
+}
+
 // acc_add onto an accumulator known to hold zero (iteration 0: init zeroes
 // them): the same arithmetic on a zero operand, with no read.
 __device__ __forceinline__ void acc_add_to_zero(void *base, uint32_t mode, uint32_t v, double x) {
@@ -359,15 +404,16 @@ __device__ __forceinline__ uint64_t static_waves_per_xcd() {
   return uint64_t(gridDim.x / 8) * (blockDim.x / 64);
 }
 
-__device__ __forceinline__ uint64_t grab_chunk(uint32_t *work, uint64_t chunks) {
-  const uint64_t per = (chunks + 7) / 8;
+// The same dealing for any work unit: `items` items, S of them static per
+// XCD, the caller's static index idx.
+__device__ __forceinline__ uint64_t grab_item(uint32_t *work, uint64_t items, uint64_t S) {
+  const uint64_t per = (items + 7) / 8;
   const uint32_t home = blockIdx.x % 8;
-  const uint64_t S = static_waves_per_xcd();
   for (uint32_t k = 0; k < 8; k++) {
     const uint32_t q = (home + k) % 8;
     const uint64_t base = q * per;
-    if (base >= chunks) continue;
-    const uint64_t lim = per < chunks - base ? per : chunks - base;
+    if (base >= items) continue;
+    const uint64_t lim = per < items - base ? per : items - base;
     if (S >= lim) continue;
     uint32_t *ctr = work + q * kWorkStride;
     if (S + __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= lim) continue;
@@ -377,13 +423,21 @@ __device__ __forceinline__ uint64_t grab_chunk(uint32_t *work, uint64_t chunks) 
   return UINT64_MAX;
 }
 
-__device__ __forceinline__ uint64_t first_chunk(uint32_t *work, uint64_t chunks) {
-  const uint64_t per = (chunks + 7) / 8;
+__device__ __forceinline__ uint64_t first_item(uint32_t *work, uint64_t items, uint64_t S,
+                                               uint64_t idx) {
+  const uint64_t per = (items + 7) / 8;
   const uint64_t base = (blockIdx.x % 8) * per;
+  if (idx < S && base < items && idx < items - base && idx < per) return base + idx;
+  return grab_item(work, items, S);
+}
+
+__device__ __forceinline__ uint64_t grab_chunk(uint32_t *work, uint64_t chunks) {
+  return grab_item(work, chunks, static_waves_per_xcd());
+}
+
+__device__ __forceinline__ uint64_t first_chunk(uint32_t *work, uint64_t chunks) {
   const uint64_t idx = uint64_t(blockIdx.x / 8) * (blockDim.x / 64) + threadIdx.x / 64;
-  if (idx < static_waves_per_xcd() && base < chunks && idx < chunks - base && idx < per)
-    return base + idx;
-  return grab_chunk(work, chunks);
+  return first_item(work, chunks, static_waves_per_xcd(), idx);
 }
 
 // Orders the LDS traffic of the W lanes of a lane group (W <= 64, one
@@ -444,14 +498,20 @@ __device__ __forceinline__ void seed_register(uint32_t hash, uint64_t seed, uint
 // the lane owning that register's slab takes the maximum.  No successor
 // counter or change mask is read.
 //
-// MODE: 0 whole counters, 1 delta gathering, 2 seeded counters.
+// Untested (MODE 3, window iterations): every successor other than v is
+// merged, modified or not.  This is exact when the iteration checks every
+// node, since each counter already dominates its successors' counters of the
+// last iteration (the invariant above), so an unmodified successor raises
+// nothing; the caller counts visited arcs from in-degrees instead.
+//
+// MODE: 0 whole counters, 1 delta gathering, 2 seeded counters, 3 untested.
 template <int R, int G, int SPL, int MODE, int NSEG = 1>
 __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int gl,
                                                  uint64_t first, uint64_t end, uint64_t stride,
                                                  uint32_t *prev, uint32_t *cur, bool &loaded,
                                                  uint32_t &nvisit, uint32_t &acc_gathered,
                                                  uint32_t *ids, int seg = 0) {
-  constexpr bool DELTA = MODE == 1, SEEDED = MODE == 2;
+  constexpr bool DELTA = MODE == 1, SEEDED = MODE == 2, UNTESTED = MODE == 3;
   constexpr int D = R * SPL;         // dwords per lane
   constexpr int CD = G * D;          // dwords per counter segment
   constexpr int K = ids_per_lane<G>();
@@ -486,7 +546,7 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
 #pragma unroll
       for (int j = 0; j < K; j++) {
         // seeded counters: init marks every node modified
-        f[j] = (s[j] != v) && (SEEDED || test_bit(a.curr_mod, s[j]));
+        f[j] = (s[j] != v) && (SEEDED || UNTESTED || test_bit(a.curr_mod, s[j]));
         mk[j] = ~0u;
       }
       if (SEEDED) {
@@ -865,18 +925,20 @@ __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, ui
   lane_sync<64>();  // the next chunk's staging overwrites meta
 }
 
-// Rows longer than heavy_degree: the whole workgroup works on one node.  Its
-// kBlock/G lane groups take interleaved slices of the row, keep private
-// maxima, and fold them in LDS; lane group 0 finishes the node.  Every thread
-// of the block calls it with the same v.  LDS: s_acc holds kBlock * D words,
-// s_ids 2 * kBlock * ids_per_lane<G>() words (a slot per lane group, as in the
-// chunk phase), s_sum two words (any lane group loaded the counter; visited
-// arcs).
-template <int R, int G, int SPL>
+// Rows longer than heavy_degree: the whole workgroup (NT threads) works on
+// one node.  Its NT/G lane groups take interleaved slices of the row, keep
+// private maxima, and fold them in LDS; lane group 0 finishes the node.
+// Every thread of the block calls it with the same v.  LDS: s_acc holds
+// NT * D words, s_ids 2 * NT * ids_per_lane<G>() words (a slot per lane
+// group, as in the chunk phase), s_sum two words (any lane group loaded the
+// counter; visited arcs).  Without COUNT the node's visited arcs and gathered
+// slabs are not counted (the window kernel counts them from in-degrees), and
+// under F_UNTESTED its successors are merged untested (merge_slices MODE 3).
+template <int R, int G, int SPL, int NT = kBlock, bool COUNT = true>
 __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc,
                                            uint32_t *s_ids, uint32_t *s_sum,
                                            Acc &acc) {
-  constexpr int D = R * SPL, NG = kBlock / G, CD = G * D;
+  constexpr int D = R * SPL, NG = NT / G, CD = G * D;
   const int g = threadIdx.x / G, gl = threadIdx.x % G;
   const bool in_list = !(a.flags & F_LIST) || test_bit(a.must, v) || test_bit(a.curr_mod, v);
   const bool check = (a.flags & F_CHECK_ALL) || test_bit(a.must, v);
@@ -890,8 +952,13 @@ __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint3
     const uint64_t first = beg + uint64_t(g) * slice_width<G>();
     const uint64_t stride = uint64_t(NG) * slice_width<G>();
     uint32_t *slot = s_ids + g * 2 * slice_width<G>();
-    merge_successors<R, G, SPL>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
-                                gathered, slot);
+    if (!COUNT && (a.flags & F_UNTESTED))
+      merge_slices<R, G, SPL, 3>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                 gathered, slot);
+    else
+      merge_successors<R, G, SPL>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                  gathered, slot);
+    if (!COUNT) nvisit = gathered = 0;  // the caller counts them from in-degrees
   }
   acc.gathered += gathered;
   if (threadIdx.x == 0) s_sum[0] = s_sum[1] = 0;
@@ -999,6 +1066,369 @@ void union_kernel(UnionArgs a) {
     c = __shfl(c, 0);
   }
   flush_stats(acc, a.stats, a.stat_slots);
+}
+
+// ---- Window kernel: dense iterations with 64-register counters ------------
+//
+// With log2m 6 a counter is 40 (r = 5) or 48 (r = 6) bytes, and a random
+// counter read moves a whole 128-byte line: in a dense iteration of
+// union_kernel nearly every successor read misses the L2, so the step moves
+// about three times its algorithmic bytes.  Web graphs take most successors
+// from nearby ids.  Here a workgroup takes a block of consecutive nodes and
+// first copies the current counters of [b0 - h, b0 + block + h) (the tile),
+// their modified bits and the block's row offsets into LDS with coalesced
+// reads; successors inside the tile are merged from LDS, the others read from
+// a.curr, and a node's own counter comes from the tile.
+//
+// Work inside a block: rows longer than kWinMedium ids take a wavefront each
+// (its 32 lane groups take interleaved runs of ids and fold their maxima
+// across lanes), the others a lane group each, in rounds of 32 nodes sorted
+// by out-degree (a counting sort per block), so the lane groups of a round
+// walk rows of the same length; wavefronts take the wavefront rows first,
+// then the rounds (longest rows first), from an LDS counter.  Rows longer than
+// a.heavy_degree take the whole workgroup each before the blocks
+// (heavy_node).  Blocks are dealt per XCD in node order (first_item /
+// grab_item), so the workgroups of an XCD work on neighbouring blocks and
+// successors just outside a tile tend to hit that XCD's L2.
+//
+// Results are those of union_kernel: the merge is a per-register maximum,
+// and every statistic is an integer sum.
+#ifndef HB_WIN_NT
+#define HB_WIN_NT 1024
+#endif
+#ifndef HB_WIN_B
+#define HB_WIN_B 1024
+#endif
+#ifndef HB_WIN_LDS
+#define HB_WIN_LDS 163840
+#endif
+constexpr int kWinNT = HB_WIN_NT;   // threads per workgroup
+constexpr int kWinB = HB_WIN_B;     // nodes per block, at most
+constexpr int kWinMedium = 62;      // longer rows take a wavefront each
+constexpr int kWinClasses = kWinMedium + 2;  // wavefront rows, then light rows by
+                                              // decreasing out-degree (kWinMedium .. 0):
+                                              // one class per lane of a wavefront
+static_assert(kWinNT % 64 == 0 && kWinB % 32 == 0 && kWinB <= kWinNT, "window kernel shape");
+
+template <int R>
+struct WinLds {
+  static constexpr int kCdw = 2 * R;  // dwords of a 64-register counter
+  static constexpr int kFixed = 8 * (kWinB + 1) + 2 * kWinB + 8 * kWinClasses + 64;
+  // tile counters: what the LDS budget leaves (with a modified bit each and
+  // two spare bitmap words), a multiple of 32
+  static constexpr int kTile = ((HB_WIN_LDS - kFixed - 16) * 8 / (32 * kCdw + 1)) / 32 * 32;
+  union U {
+    uint32_t tile[kTile * kCdw];
+    struct {
+      uint32_t acc[kWinNT * R];
+      uint32_t ids[2 * kWinNT * ids_per_lane<2>()];
+      uint32_t sum[2];
+    } heavy;
+  } u;
+  uint32_t tmod[kTile / 32 + 2];   // curr_mod words from the tile's first node's word on
+  uint64_t off[kWinB + 1];         // row offsets of the block
+  uint16_t list[kWinB];            // block positions, by class
+  uint32_t cls[kWinClasses];       // nodes per class
+  uint32_t base[kWinClasses + 1];  // first list position of each class
+  uint32_t q;                      // next work item
+  uint64_t item;                   // block or heavy row being worked on
+};
+
+// Merges into cur the counters of v's successors (other than v) among the
+// ids [base, base + WU) for base = first, first + stride, ... below end
+// (hyperball_impl.rs:1037-1049): from the tile (LDS) when inside
+// [tlo, tlo + tlen), else from a.curr.  Both lanes of the group read the same
+// ids and merge the same successors, each into its own slab; WU loads are in
+// flight per lane, and the next run's ids are read before this run's
+// counters are merged.  Successors are merged untested, modified or not
+// (exact when every node is checked: merge_slices MODE 3); with TEST an
+// outside successor is read only when its modified bit is set, which saves
+// its counter read in iterations where few nodes changed.
+#ifndef HB_WIN_U
+#define HB_WIN_U 4
+#endif
+template <int R, bool TEST>
+__device__ __forceinline__ void win_merge(const UnionArgs &a, uint32_t v, int gl, uint64_t first,
+                                          uint64_t end, uint64_t stride, const uint32_t *tile,
+                                          uint64_t tlo, uint32_t tlen, uint32_t *cur) {
+  constexpr int D = R, CD = 2 * R, WU = HB_WIN_U;  // loads in flight per lane
+  constexpr uint32_t CS = counter_stride(CD);
+  const uint32_t *gbase = a.curr + gl * D;
+  const uint32_t *lbase = tile + gl * D;
+  uint32_t nx[WU];
+#pragma unroll
+  for (int u = 0; u < WU; u++) nx[u] = first + u < end ? a.succ[first + u] : v;
+#pragma unroll 1
+  for (uint64_t base = first; base < end; base += stride) {
+    uint32_t ids[WU];
+#pragma unroll
+    for (int u = 0; u < WU; u++) ids[u] = nx[u];
+    const uint64_t nb = base + stride;
+#pragma unroll
+    for (int u = 0; u < WU; u++) nx[u] = nb + u < end ? a.succ[nb + u] : v;
+    uint32_t y[WU][D];
+#pragma unroll
+    for (int u = 0; u < WU; u++) {
+      const uint32_t t = ids[u];
+      const uint64_t ti = uint64_t(t) - tlo;  // wraps above tlen when t < tlo
+      const bool in = ti < tlen;
+      const bool f = t != v && (in || !TEST || test_bit(a.curr_mod, t));
+      if (f && in) {
+        const uint32_t *p = lbase + ti * CD;
+#pragma unroll
+        for (int b = 0; b < D; b++) y[u][b] = p[b];
+      } else if (f) {
+        load_part<D>(y[u], gbase + uint64_t(t) * CS);
+      } else {
+#pragma unroll
+        for (int b = 0; b < D; b++) y[u][b] = 0u;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < WU; u++) slab_max<R>(cur, y[u]);
+  }
+}
+
+// Node data that the end of a node's body needs, read by lane 0 of its group
+// before the merge so that the reads overlap it: the cached count of the
+// current counter and the raw sum-of-distances and harmonic accumulators.
+struct WinNode {
+  double pre;
+  uint2 sd, hc;
+};
+
+__device__ __forceinline__ WinNode win_prefetch(const UnionArgs &a, uint32_t v) {
+  WinNode w;
+  w.pre = a.cache[v];
+  w.sd = w.hc = make_uint2(0u, 0u);
+  if (a.flags & F_CENTRALITY) {
+    if (a.sd) w.sd = acc_load(a.sd, a.acc_mode, v);
+    if (a.hc) w.hc = acc_load(a.hc, a.acc_mode, v);
+  }
+  return w;
+}
+
+// The end of the node body (hyperball_impl.rs:1051-1144) for a checked node
+// of a window iteration (not systolic, counters read, visited arcs counted
+// elsewhere): node_update and the store of finish_node, with the node data of
+// win_prefetch (valid on lane 0 of the group).
+template <int R>
+__device__ __forceinline__ void win_finish(const UnionArgs &a, uint32_t v, int gl, bool was_mod,
+                                           const uint32_t *prev, const uint32_t *cur,
+                                           uint64_t deg, const WinNode &w, Acc &acc) {
+  constexpr int D = R;
+  constexpr uint32_t CS = counter_stride(2 * R);
+  uint32_t d = 0;
+#pragma unroll
+  for (int b = 0; b < D; b++) d |= cur[b] ^ prev[b];
+  const bool cm = group_ballot<2>(d != 0) != 0;
+  double post = 0.0;
+  uint32_t chg_bits = 0;
+  if (cm) {
+    chg_bits = group_or_u32<2>(d ? 1u << (uint32_t(gl) >> a.chg_shift) : 0u);
+    post = group_count<R, 2, 1>(cur, a.alpha_m_m, a.m, a.lc_table);
+  }
+  if (gl == 0) {
+    if (cm) {
+      a.chg_next[v] = chg_bits;
+      acc.changed += __popc(chg_bits);
+      a.cache[v] = post;
+      acc.add_nf(post);
+      if (a.flags & F_CENTRALITY) {
+        const double delta = post - w.pre;
+        if (delta > 0.0) {
+          if (a.sd) acc_store(a.sd, a.acc_mode, v, acc_apply(w.sd, a.acc_mode, delta * a.dist));
+          if (a.hc) acc_store(a.hc, a.acc_mode, v, acc_apply(w.hc, a.acc_mode, delta / a.dist));
+          for (int k = 0; k < a.ndisc; k++) acc_add(a.dc[k], a.acc_mode, v, delta * a.disc[k]);
+        }
+      }
+      atomicOr(a.next_mod + (v >> 5), 1u << (v & 31));
+      acc.modified++;
+    } else {
+      acc.add_nf(w.pre);
+    }
+    acc.scanned += deg;
+    acc.checked++;
+  }
+  if (cm || was_mod) store_part<D>(a.next + uint64_t(v) * CS + gl * D, cur);
+}
+
+// A value the whole wavefront holds (read from LDS after a barrier), moved to
+// scalar registers.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+  return uint64_t(hi) << 32 | lo;
+}
+
+template <int R, bool TEST>
+__global__ __launch_bounds__(kWinNT) void window_kernel(UnionArgs a) {
+  using LT = WinLds<R>;
+  constexpr int D = R, CD = 2 * R, T = LT::kTile;
+  constexpr uint32_t CS = counter_stride(CD);
+  static_assert(T >= kWinB + 64, "the tile holds the block");
+  static_assert(sizeof(typename LT::U) >= sizeof(uint32_t) * (kWinNT * (R + 2 * ids_per_lane<2>()) + 2),
+                "the heavy rows' buffers fit the tile");
+  __shared__ LT L;
+  Acc acc;
+  const int tid = threadIdx.x, g = __lane_id() / 2, gl = __lane_id() & 1;
+  if (blockIdx.x == 0 && tid == 0) a.stats->list_len = a.count;
+
+  // rows longer than heavy_degree: the whole workgroup each
+  if (a.heavy_count) {
+    for (;;) {
+      if (tid == 0) L.item = atomicAdd(a.work + 8 * kWorkStride, 1u);
+      __syncthreads();
+      const uint64_t h = L.item;
+      __syncthreads();
+      if (h >= a.heavy_count) break;
+      heavy_node<R, 2, 1, kWinNT, false>(a, a.heavy_list[h], L.u.heavy.acc, L.u.heavy.ids,
+                                         L.u.heavy.sum, acc);
+    }
+  }
+
+  const uint64_t n = a.count;
+  const uint32_t B = a.win_block && a.win_block < uint32_t(kWinB) ? a.win_block : uint32_t(kWinB);
+  const uint32_t hmax = uint32_t(T - B) / 2;
+  const uint32_t half = a.win_half && a.win_half < hmax ? a.win_half : hmax;
+  const uint64_t blocks = (n + B - 1) / B;
+  const uint64_t S = gridDim.x / 8;
+  static_assert(kWinClasses == 64 && kWinNT >= 64, "one class per lane of the first wavefront");
+  if (tid < kWinClasses) L.cls[tid] = 0;
+  if (tid == 0) {
+    L.item = first_item(a.work, blocks, S, blockIdx.x / 8);
+    L.q = 0;
+  }
+  __syncthreads();
+  for (;;) {
+    const uint64_t blk = uniform_u64(L.item);
+    if (blk == UINT64_MAX) break;
+    const uint64_t b0 = blk * B;
+    const uint32_t bn = uint32_t(n - b0 < B ? n - b0 : B);
+    const uint64_t tlo = b0 > half ? b0 - half : 0;
+    const uint64_t thi = b0 + bn + half < n ? b0 + bn + half : n;
+    const uint32_t tlen = uint32_t(thi - tlo);
+    const uint64_t wlo = tlo & ~uint64_t(31);
+
+    // stage the tile (64-bit loads, all issued before the stores), its
+    // modified bits and the block's offsets; rank the block's nodes by class
+    {
+      constexpr int H = CD / 2;  // 64-bit words per counter
+      constexpr int PER = (T * H + kWinNT - 1) / kWinNT;
+      const uint2 *src = reinterpret_cast<const uint2 *>(a.curr);
+      uint2 *dst = reinterpret_cast<uint2 *>(L.u.tile);
+      const uint32_t words = tlen * uint32_t(H);
+      constexpr int STEP = 8;  // loads in flight per thread
+#pragma unroll 1
+      for (int j0 = 0; j0 < PER; j0 += STEP) {
+        uint2 w[STEP];
+#pragma unroll
+        for (int j = 0; j < STEP; j++) {
+          const uint32_t k = uint32_t((j0 + j) * kWinNT + tid);
+          w[j] = k < words ? src[(tlo + k / H) * (CS / 2) + k % H] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int j = 0; j < STEP; j++) {
+          const uint32_t k = uint32_t((j0 + j) * kWinNT + tid);
+          if (k < words) dst[k] = w[j];
+        }
+      }
+    }
+    {
+      const uint64_t w1 = (thi - 1) >> 5;
+      for (uint64_t k = tid; k <= w1 - (wlo >> 5); k += kWinNT) L.tmod[k] = a.curr_mod[(wlo >> 5) + k];
+    }
+    for (uint32_t k = tid; k <= bn; k += kWinNT) L.off[k] = a.off[b0 + k];
+    uint32_t my_cls = kWinClasses, my_rank = 0;
+    if (uint32_t(tid) < bn) {
+      const uint64_t v = b0 + tid;
+      const uint64_t deg = a.off[v + 1] - a.off[v];
+      if (deg <= a.heavy_degree) {
+        my_cls = deg > uint64_t(kWinMedium) ? 0u : uint32_t(kWinMedium + 1 - deg);
+        my_rank = atomicAdd(&L.cls[my_cls], 1u);
+      }
+      if (test_bit(a.curr_mod, uint32_t(v))) {
+        const uint32_t k = a.indeg[v];
+        acc.visited += k;
+        acc.gathered += 2 * uint64_t(k);
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {  // exclusive prefix sums of the class sizes, one class per lane
+      const uint32_t c = L.cls[tid];
+      uint32_t x = c;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (tid >= o) x += y;
+      }
+      L.base[tid] = x - c;
+      if (tid == 63) L.base[kWinClasses] = x;
+    }
+    __syncthreads();
+    if (my_cls < uint32_t(kWinClasses)) L.list[L.base[my_cls] + my_rank] = uint16_t(tid);
+    __syncthreads();
+
+    // work items: the wavefront rows, then rounds of 32 light rows
+    const uint32_t nw = __builtin_amdgcn_readfirstlane(L.cls[0]);
+    const uint32_t nl = __builtin_amdgcn_readfirstlane(L.base[kWinClasses]) - nw;
+    const uint32_t items = nw + (nl + 31) / 32;
+    for (;;) {
+      uint32_t it = 0;
+      if (__lane_id() == 0) it = atomicAdd(&L.q, 1u);
+      it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+      if (it >= items) break;
+      const bool wave_row = it < nw;
+      const uint32_t pos = wave_row ? it : nw + (it - nw) * 32 + uint32_t(g);
+      if (pos < nw + nl) {
+        const uint32_t p = L.list[pos];
+        const uint32_t v = uint32_t(b0 + p);
+        WinNode wn{};
+        if (gl == 0 && (!wave_row || g == 0)) wn = win_prefetch(a, v);
+        const uint64_t beg = L.off[p], end = L.off[p + 1];
+        const uint64_t own = uint64_t(v) - tlo;
+        uint32_t prev[D], cur[D];
+#pragma unroll
+        for (int b = 0; b < D; b++) prev[b] = cur[b] = L.u.tile[own * CD + gl * D + b];
+        if (wave_row) {
+          win_merge<R, TEST>(a, v, gl, beg + HB_WIN_U * uint64_t(g), end, HB_WIN_U * 32, L.u.tile,
+                             tlo, tlen, cur);
+#pragma unroll
+          for (int o = 2; o < 64; o <<= 1) {
+            uint32_t y[D];
+#pragma unroll
+            for (int b = 0; b < D; b++) y[b] = __shfl_xor(cur[b], o);
+            slab_max<R>(cur, y);
+          }
+        } else {
+          win_merge<R, TEST>(a, v, gl, beg, end, HB_WIN_U, L.u.tile, tlo, tlen, cur);
+        }
+        const uint64_t ob = uint64_t(v) - wlo;
+        const bool was_mod = (L.tmod[ob >> 5] >> (ob & 31)) & 1u;
+        if (!wave_row || g == 0) win_finish<R>(a, v, gl, was_mod, prev, cur, end - beg, wn, acc);
+      }
+    }
+    __syncthreads();
+    if (tid < kWinClasses) L.cls[tid] = 0;
+    if (tid == 0) {
+      L.item = grab_item(a.work, blocks, S);
+      L.q = 0;
+    }
+    __syncthreads();
+  }
+  flush_stats(acc, a.stats, a.stat_slots);
+}
+
+__global__ __launch_bounds__(kBlock) void indegree_kernel(const uint64_t *off, const uint32_t *succ,
+                                                          uint64_t n, uint32_t *indeg) {
+  const uint64_t waves = uint64_t(gridDim.x) * (kBlock / 64);
+  for (uint64_t v = uint64_t(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64; v < n; v += waves) {
+    const uint64_t e = off[v + 1];
+    for (uint64_t k = off[v] + __lane_id(); k < e; k += 64) {
+      const uint32_t s = succ[k];
+      if (s != v) atomicAdd(indeg + s, 1u);
+    }
+  }
 }
 
 // Seeding without weights, fused with the count: G lanes per node write the
@@ -1755,6 +2185,58 @@ hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hip
   HB_SHAPES(X)
 #undef X
   return hipErrorInvalidValue;
+}
+
+bool window_union_supported(int r, int log2m) { return (r == 5 || r == 6) && log2m == 6; }
+
+uint32_t window_block_nodes() { return uint32_t(kWinB); }
+
+uint32_t window_tile_nodes(int r) {
+  return r == 5 ? uint32_t(WinLds<5>::kTile) : r == 6 ? uint32_t(WinLds<6>::kTile) : 0u;
+}
+
+template <int R, bool TEST>
+static hipError_t launch_window_t(const UnionArgs &a, hipStream_t s) {
+  static std::atomic<uint32_t> cache[kMaxDevices];
+  uint32_t grid = 0;
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kMaxDevices) {
+      grid = cache[dev].load(std::memory_order_relaxed);
+      if (!grid) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, window_kernel<R, TEST>, kWinNT, 0) ==
+                hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+                hipSuccess) {
+          grid = uint32_t(std::max(1, per_cu) * std::max(1, cus));
+          cache[dev].store(grid, std::memory_order_relaxed);
+        }
+      }
+    }
+    (void)hipGetLastError();
+  }
+  const uint64_t B = a.win_block && a.win_block < uint32_t(kWinB) ? a.win_block : kWinB;
+  const uint64_t blocks = (a.count + B - 1) / B;
+  uint64_t g = grid ? grid : 256;
+  g = std::min<uint64_t>(g, std::max<uint64_t>(blocks, a.heavy_count));
+  g = std::max<uint64_t>(8, (g + 7) / 8 * 8);
+  window_kernel<R, TEST><<<uint32_t(g), kWinNT, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_window_union(int r, int log2m, const UnionArgs &a, hipStream_t s) {
+  if (!window_union_supported(r, log2m) || !a.indeg) return hipErrorInvalidValue;
+  const bool test = a.flags & F_UNTESTED ? false : true;
+  if (r == 5) return test ? launch_window_t<5, true>(a, s) : launch_window_t<5, false>(a, s);
+  return test ? launch_window_t<6, true>(a, s) : launch_window_t<6, false>(a, s);
+}
+
+hipError_t launch_indegree(const uint64_t *off, const uint32_t *succ, uint64_t n, uint32_t *indeg,
+                           hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  indegree_kernel<<<grid_for(n, kBlock / 64, 65536), kBlock, 0, s>>>(off, succ, n, indeg);
+  return hipGetLastError();
 }
 
 int batch_shape(int r, int log2m) {

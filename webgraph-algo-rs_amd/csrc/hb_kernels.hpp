@@ -40,6 +40,8 @@ enum UnionFlags : uint32_t {
   F_DELTA = 16u,      // gather only the changed slabs of successors (change masks)
   F_SEEDED = 32u,     // counters are as seeded without weights: each holds the one register
                       // of its node's hash, computed instead of read
+  F_UNTESTED = 64u,   // window kernel: successors outside the tile (and those of rows that
+                      // take a workgroup) are merged without testing their modified bits
 };
 
 enum AccMode : uint32_t { ACC_F64 = 0, ACC_F32_KAHAN = 1, ACC_F32 = 2 };
@@ -123,6 +125,11 @@ struct UnionArgs {
   const uint32_t *chg_cur;
   uint32_t *chg_next;
   uint32_t chg_shift;
+  // Window kernel (launch_window_union): nodes per block and the half-width
+  // of the counter tile around a block, at most the kernel's own (0: those);
+  // indeg[s]: arcs into s from other nodes (self loops excluded).
+  uint32_t win_block, win_half;
+  const uint32_t *indeg;
 };
 
 // Streamed counters (counter arrays in host memory, worked on tile by tile
@@ -208,6 +215,23 @@ bool union_shape(int r, int log2m, int *lanes, int *slabs_per_lane, int *segment
 // Launches union_kernel (grid workgroups, capped at the resident ones): the
 // heavy rows of a.heavy_list first, a workgroup each, then the chunks.
 hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s);
+
+// The window kernel: the union step of an iteration that checks every node
+// (F_CHECK_ALL, not F_LIST, F_SEEDED or F_DELTA) for 64-register counters.
+// A workgroup takes blocks of consecutive nodes and first copies the current
+// counters of the nodes around its block into LDS; successors there are
+// merged from LDS, the others read from a.curr.  Rows longer than
+// a.heavy_degree (a.heavy_list, a.heavy_count) take the whole workgroup each,
+// first.  window_union_supported tells the shapes it has; window_tile_nodes
+// the counters its tile holds (block + both halves).
+bool window_union_supported(int r, int log2m);
+uint32_t window_block_nodes();
+uint32_t window_tile_nodes(int r);
+hipError_t launch_window_union(int r, int log2m, const UnionArgs &a, hipStream_t s);
+// indeg[s] <- the arcs (v, s) with v != s of the CSR graph (off, succ); indeg
+// must be zeroed.
+hipError_t launch_indegree(const uint64_t *off, const uint32_t *succ, uint64_t n, uint32_t *indeg,
+                           hipStream_t s);
 hipError_t launch_count_all(int r, int log2m, const uint32_t *counters, uint64_t n,
                             double alpha_m_m, uint32_t m, const double *lc_table, double *out,
                             hipStream_t s);

@@ -86,10 +86,11 @@ class IterationStats(ctypes.Structure):
                 ("copied", ctypes.c_uint64), ("modified", ctypes.c_uint64), ("nf", ctypes.c_double),
                 ("union_ms", ctypes.c_double), ("iteration_ms", ctypes.c_double),
                 ("gathered_slabs", ctypes.c_uint64), ("delta_gathering", ctypes.c_uint32),
-                ("streamed", ctypes.c_uint32)]
+                ("streamed", ctypes.c_uint32), ("window", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 DISCOUNT_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p)
@@ -149,6 +150,7 @@ _SIGNATURES = [
     ("hb_batch_set_graph", _I, [_P, _I]),
     ("hb_autotune", _I, [_P, _U32, _P, _P, _SZ, _P, _P]),
     ("hb_set_delta_gathering", _I, [_P, _I]),
+    ("hb_set_window", _I, [_P, _I, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("hb_set_streaming", _I, [_P, _I]),
     ("hb_set_progress", _I, [_P, _P, _P]),
     ("hb_set_stream", _I, [_P, _P]),
@@ -620,6 +622,15 @@ class HyperBall:
     def set_delta_gathering(self, mode: int) -> None:
         """0: per iteration by the library's rule, 1: always, 2: never."""
         _check(lib().hb_set_delta_gathering(self._h, int(mode)))
+
+    def set_window(self, mode: int, block_nodes: int = 0, half_width: int = 0,
+                   heavy_degree: int = 0) -> None:
+        """Window kernel for log2m-6 counters: 0 or 1 in the iterations that
+        check every node, 2 never; block_nodes / half_width cap its block and
+        tile (0: the kernel's own); rows longer than heavy_degree take a
+        workgroup (0: the default, 2**32 - 1: none) (hb_set_window)."""
+        _check(lib().hb_set_window(self._h, int(mode), int(block_nodes), int(half_width),
+                                   int(heavy_degree)))
 
     def set_streaming(self, mode: int) -> None:
         """Streamed counters: 0 by the per-iteration rule, 1 always streamed,
