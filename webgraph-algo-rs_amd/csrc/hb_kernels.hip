@@ -1141,9 +1141,11 @@ struct WinLds {
 // ids and merge the same successors, each into its own slab; WU loads are in
 // flight per lane, and the next run's ids are read before this run's
 // counters are merged.  Successors are merged untested, modified or not
-// (exact when every node is checked: merge_slices MODE 3); with TEST an
-// outside successor is read only when its modified bit is set, which saves
-// its counter read in iterations where few nodes changed.
+// (exact when every node is checked: merge_slices MODE 3), and so is v
+// itself (a self loop, or a slot past the row end), which changes nothing;
+// with TEST an outside successor is read only when its modified bit is set,
+// which saves its counter read in iterations where few nodes changed.  The
+// tile holds v (its block lies inside the tile).
 #ifndef HB_WIN_U
 #define HB_WIN_U 4
 #endif
@@ -1155,6 +1157,9 @@ __device__ __forceinline__ void win_merge(const UnionArgs &a, uint32_t v, int gl
   constexpr uint32_t CS = counter_stride(CD);
   const uint32_t *gbase = a.curr + gl * D;
   const uint32_t *lbase = tile + gl * D;
+  const uint32_t self = v - uint32_t(tlo);  // v's place in the tile
+  // ids past the row read v itself: its own counter (in the tile) changes
+  // nothing, and neither does a self loop
   uint32_t nx[WU];
 #pragma unroll
   for (int u = 0; u < WU; u++) nx[u] = first + u < end ? a.succ[first + u] : v;
@@ -1166,56 +1171,56 @@ __device__ __forceinline__ void win_merge(const UnionArgs &a, uint32_t v, int gl
     const uint64_t nb = base + stride;
 #pragma unroll
     for (int u = 0; u < WU; u++) nx[u] = nb + u < end ? a.succ[nb + u] : v;
-    uint32_t y[WU][D];
+    // The HBM reads of the run first, all in flight together, into registers
+    // of their own: a tile read into the same registers would have to wait
+    // for them.  Then the tile reads, one successor at a time, merged with
+    // whichever of the two holds its counter; lanes whose successor lies
+    // outside read v's own counter from the tile, which changes nothing.
+    uint32_t yg[WU][D];
 #pragma unroll
     for (int u = 0; u < WU; u++) {
       const uint32_t t = ids[u];
-      const uint64_t ti = uint64_t(t) - tlo;  // wraps above tlen when t < tlo
-      const bool in = ti < tlen;
-      const bool f = t != v && (in || !TEST || test_bit(a.curr_mod, t));
-      if (f && in) {
-        const uint32_t *p = lbase + ti * CD;
+      // tlo + tlen <= n <= 2^32, so t - tlo wraps to at least tlen when t < tlo
+      const bool out = t - uint32_t(tlo) >= tlen;
 #pragma unroll
-        for (int b = 0; b < D; b++) y[u][b] = p[b];
-      } else if (f) {
-        load_part<D>(y[u], gbase + uint64_t(t) * CS);
-      } else {
-#pragma unroll
-        for (int b = 0; b < D; b++) y[u][b] = 0u;
-      }
+      for (int b = 0; b < D; b++) yg[u][b] = 0u;
+      if (out && (!TEST || test_bit(a.curr_mod, t))) load_part<D>(yg[u], gbase + uint64_t(t) * CS);
     }
 #pragma unroll
-    for (int u = 0; u < WU; u++) slab_max<R>(cur, y[u]);
+    for (int u = 0; u < WU; u++) {
+      const uint32_t ti = ids[u] - uint32_t(tlo);
+      const bool in = ti < tlen;
+      const uint32_t *p = lbase + (in ? ti : self) * CD;
+      uint32_t y[D];
+#pragma unroll
+      for (int b = 0; b < D; b++) y[b] = in ? p[b] : yg[u][b];
+      slab_max<R>(cur, y);
+    }
   }
 }
 
 // Node data that the end of a node's body needs, read by lane 0 of its group
-// before the merge so that the reads overlap it: the cached count of the
-// current counter and the raw sum-of-distances and harmonic accumulators.
+// before the merge so that the read overlaps it: the cached count of the
+// current counter.
 struct WinNode {
   double pre;
-  uint2 sd, hc;
 };
 
 __device__ __forceinline__ WinNode win_prefetch(const UnionArgs &a, uint32_t v) {
   WinNode w;
   w.pre = a.cache[v];
-  w.sd = w.hc = make_uint2(0u, 0u);
-  if (a.flags & F_CENTRALITY) {
-    if (a.sd) w.sd = acc_load(a.sd, a.acc_mode, v);
-    if (a.hc) w.hc = acc_load(a.hc, a.acc_mode, v);
-  }
   return w;
 }
 
 // The end of the node body (hyperball_impl.rs:1051-1144) for a checked node
-// of a window iteration (not systolic, counters read, visited arcs counted
-// elsewhere): node_update and the store of finish_node, with the node data of
-// win_prefetch (valid on lane 0 of the group).
+// of a window iteration (not systolic, counters read; visited arcs, scanned
+// arcs and checked nodes counted when the block is staged): node_update and
+// the store of finish_node, with the node data of win_prefetch (valid on
+// lane 0 of the group).
 template <int R>
 __device__ __forceinline__ void win_finish(const UnionArgs &a, uint32_t v, int gl, bool was_mod,
                                            const uint32_t *prev, const uint32_t *cur,
-                                           uint64_t deg, const WinNode &w, Acc &acc) {
+                                           const WinNode &w, Acc &acc) {
   constexpr int D = R;
   constexpr uint32_t CS = counter_stride(2 * R);
   uint32_t d = 0;
@@ -1237,8 +1242,8 @@ __device__ __forceinline__ void win_finish(const UnionArgs &a, uint32_t v, int g
       if (a.flags & F_CENTRALITY) {
         const double delta = post - w.pre;
         if (delta > 0.0) {
-          if (a.sd) acc_store(a.sd, a.acc_mode, v, acc_apply(w.sd, a.acc_mode, delta * a.dist));
-          if (a.hc) acc_store(a.hc, a.acc_mode, v, acc_apply(w.hc, a.acc_mode, delta / a.dist));
+          if (a.sd) acc_add(a.sd, a.acc_mode, v, delta * a.dist);
+          if (a.hc) acc_add(a.hc, a.acc_mode, v, delta / a.dist);
           for (int k = 0; k < a.ndisc; k++) acc_add(a.dc[k], a.acc_mode, v, delta * a.disc[k]);
         }
       }
@@ -1247,8 +1252,6 @@ __device__ __forceinline__ void win_finish(const UnionArgs &a, uint32_t v, int g
     } else {
       acc.add_nf(w.pre);
     }
-    acc.scanned += deg;
-    acc.checked++;
   }
   if (cm || was_mod) store_part<D>(a.next + uint64_t(v) * CS + gl * D, cur);
 }
@@ -1346,6 +1349,8 @@ __global__ __launch_bounds__(kWinNT) void window_kernel(UnionArgs a) {
       if (deg <= a.heavy_degree) {
         my_cls = deg > uint64_t(kWinMedium) ? 0u : uint32_t(kWinMedium + 1 - deg);
         my_rank = atomicAdd(&L.cls[my_cls], 1u);
+        acc.scanned += deg;
+        acc.checked++;
       }
       if (test_bit(a.curr_mod, uint32_t(v))) {
         const uint32_t k = a.indeg[v];
@@ -1405,7 +1410,7 @@ __global__ __launch_bounds__(kWinNT) void window_kernel(UnionArgs a) {
         }
         const uint64_t ob = uint64_t(v) - wlo;
         const bool was_mod = (L.tmod[ob >> 5] >> (ob & 31)) & 1u;
-        if (!wave_row || g == 0) win_finish<R>(a, v, gl, was_mod, prev, cur, end - beg, wn, acc);
+        if (!wave_row || g == 0) win_finish<R>(a, v, gl, was_mod, prev, cur, wn, acc);
       }
     }
     __syncthreads();
