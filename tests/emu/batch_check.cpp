@@ -113,8 +113,15 @@ int main(int argc, char **argv) {
     const Out os = outputs(s, words, p.num_nodes);
     check(hb_destroy(s), "hb_destroy");
     const bool same = ob.c0 == os.c0 && ob.mod == os.mod && ob.nf == os.nf && ob.sd == os.sd;
-    std::printf("instance %d: %s (%zu iterations)\n", i, same ? "identical" : "DIFFERENT",
-                ob.nf.size() - 1);
+    uint64_t iters = 0, windows = 0;
+    check(hb_num_iterations(hb, &iters), "hb_num_iterations");
+    for (uint64_t it = 0; it < iters; it++) {
+      hb_iteration_stats st;
+      check(hb_get_iteration_stats(hb, it, &st), "hb_get_iteration_stats");
+      windows += st.window;
+    }
+    std::printf("instance %d: %s (%zu iterations, %llu window)\n", i,
+                same ? "identical" : "DIFFERENT", ob.nf.size() - 1, (unsigned long long)windows);
     bad += !same;
   }
   check(hb_batch_destroy(b), "hb_batch_destroy");

@@ -9,6 +9,7 @@ The batch mixes all six batched shapes (log2m 6, 8, 10 with 5- and 6-bit
 registers), so wavefronts of one workgroup run different shapes at once;
 rows up to 600 successors exercise the workgroup-per-node kernel."""
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -37,6 +38,8 @@ def test_emulated_batch_matches_single_instances(emu_exes, graph, heavy, tune):
     r = subprocess.run(args, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:] + r.stderr[-6000:]
     assert r.stdout.count("identical") == 6
+    # the log2m-6 instances run their dense iterations as batched window iterations
+    assert any(int(w) > 0 for w in re.findall(r"(\d+) window\)", r.stdout))
 
 
 @pytest.mark.parametrize("log2m,r,n,dmax,seed,heavy,delta", [

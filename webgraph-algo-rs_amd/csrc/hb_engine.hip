@@ -778,7 +778,7 @@ struct Batch {
   DevBuf<uint64_t> slots;  // per instance: kStatSlots statistics slots
   Stats *h_stats = nullptr;     // pinned
   DevBuf<uint32_t> work;
-  DevBuf<uint64_t> total_chunks;
+  DevBuf<uint64_t> total_chunks, total_window;
   DevBuf<uint32_t> shape_inst, shape_off;  // instances grouped by shape
   BatchLaunch L{};
   bool use_graph = true;
@@ -896,7 +896,10 @@ struct Batch {
         continue;
       }
       any = true;
+      const uint64_t modified_prev = inst[i]->modified;
       plans[i] = inst[i]->plan_iteration();
+      inst[i]->window_now = inst[i]->plan_window(plans[i], modified_prev);
+      d.window = inst[i]->window_now;
       d.u = plans[i].ua;
       d.u.stats = d_stats.p + i;
       d.u.work = work.p;
@@ -1631,6 +1634,7 @@ hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transpos
     b->slots.alloc(k * size_t(kStatSlots) * kStatSlotWords);
     b->work.alloc(kBatchWorkWords);
     b->total_chunks.alloc(kBatchShapes);
+    b->total_window.alloc(2 * kBatchShapes);
     for (size_t i = 0; i < k; i++) {
       Instance &I = *b->inst[i];
       BatchDesc &d = b->h_desc[i];
@@ -1657,6 +1661,7 @@ hb_status hb_batch_create(size_t k, const hb_csr *graphs, const hb_csr *transpos
     L.stats = b->d_stats.p;
     L.work = b->work.p;
     L.total_chunks = b->total_chunks.p;
+    L.total_window = b->total_window.p;
     L.shape_inst = b->shape_inst.p;
     L.shape_off = b->shape_off.p;
     for (int sh = 0; sh < kBatchShapes; sh++) {

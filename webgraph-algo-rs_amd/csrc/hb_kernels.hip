@@ -1264,17 +1264,165 @@ __device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
   return uint64_t(hi) << 32 | lo;
 }
 
+// Window geometry of an instance: nodes per block (at most kWinB) and the
+// tile's half-width around a block (at most what the tile leaves).
+struct WinGeom {
+  uint32_t B, half;
+  uint64_t blocks;
+};
+
+template <int R>
+__device__ __forceinline__ WinGeom win_geom(const UnionArgs &a) {
+  constexpr int T = WinLds<R>::kTile;
+  WinGeom w;
+  w.B = a.win_block && a.win_block < uint32_t(kWinB) ? a.win_block : uint32_t(kWinB);
+  const uint32_t hmax = uint32_t(T - w.B) / 2;
+  w.half = a.win_half && a.win_half < hmax ? a.win_half : hmax;
+  w.blocks = (a.count + w.B - 1) / w.B;
+  return w;
+}
+
+// Rows longer than a.heavy_degree: heavy row h of a.heavy_list, the whole
+// workgroup (visited arcs counted at staging, as for every node).
+template <int R>
+__device__ __forceinline__ void win_heavy_row(const UnionArgs &a, uint64_t h, WinLds<R> &L,
+                                              Acc &acc) {
+  heavy_node<R, 2, 1, kWinNT, false>(a, a.heavy_list[h], L.u.heavy.acc, L.u.heavy.ids,
+                                     L.u.heavy.sum, acc);
+}
+
+// Block blk of the instance a: stage its tile, rank its nodes, work on them.
+// Every thread of the workgroup calls it; L.cls and L.q are zero on entry and
+// on return, and the workgroup has passed a barrier after the work.
+template <int R, bool TEST>
+__device__ __forceinline__ void win_block(const UnionArgs &a, uint64_t blk, const WinGeom &geo,
+                                          WinLds<R> &L, Acc &acc) {
+  constexpr int D = R, CD = 2 * R, T = WinLds<R>::kTile;
+  constexpr uint32_t CS = counter_stride(CD);
+  const int tid = threadIdx.x, g = __lane_id() / 2, gl = __lane_id() & 1;
+  const uint64_t n = a.count;
+  const uint64_t b0 = blk * geo.B;
+  const uint32_t bn = uint32_t(n - b0 < geo.B ? n - b0 : geo.B);
+  const uint64_t tlo = b0 > geo.half ? b0 - geo.half : 0;
+  const uint64_t thi = b0 + bn + geo.half < n ? b0 + bn + geo.half : n;
+  const uint32_t tlen = uint32_t(thi - tlo);
+  const uint64_t wlo = tlo & ~uint64_t(31);
+
+  // stage the tile (64-bit loads, eight in flight per thread), its modified
+  // bits and the block's offsets; rank the block's nodes by class
+  {
+    constexpr int H = CD / 2;  // 64-bit words per counter
+    constexpr int PER = (T * H + kWinNT - 1) / kWinNT;
+    const uint2 *src = reinterpret_cast<const uint2 *>(a.curr);
+    uint2 *dst = reinterpret_cast<uint2 *>(L.u.tile);
+    const uint32_t words = tlen * uint32_t(H);
+    constexpr int STEP = 8;
+#pragma unroll 1
+    for (int j0 = 0; j0 < PER; j0 += STEP) {
+      uint2 w[STEP];
+#pragma unroll
+      for (int j = 0; j < STEP; j++) {
+        const uint32_t k = uint32_t((j0 + j) * kWinNT + tid);
+        w[j] = k < words ? src[(tlo + k / H) * (CS / 2) + k % H] : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int j = 0; j < STEP; j++) {
+        const uint32_t k = uint32_t((j0 + j) * kWinNT + tid);
+        if (k < words) dst[k] = w[j];
+      }
+    }
+  }
+  {
+    const uint64_t w1 = (thi - 1) >> 5;
+    for (uint64_t k = tid; k <= w1 - (wlo >> 5); k += kWinNT) L.tmod[k] = a.curr_mod[(wlo >> 5) + k];
+  }
+  for (uint32_t k = tid; k <= bn; k += kWinNT) L.off[k] = a.off[b0 + k];
+  uint32_t my_cls = kWinClasses, my_rank = 0;
+  if (uint32_t(tid) < bn) {
+    const uint64_t v = b0 + tid;
+    const uint64_t deg = a.off[v + 1] - a.off[v];
+    if (deg <= a.heavy_degree) {
+      my_cls = deg > uint64_t(kWinMedium) ? 0u : uint32_t(kWinMedium + 1 - deg);
+      my_rank = atomicAdd(&L.cls[my_cls], 1u);
+      acc.scanned += deg;
+      acc.checked++;
+    }
+    if (test_bit(a.curr_mod, uint32_t(v))) {
+      const uint32_t k = a.indeg[v];
+      acc.visited += k;
+      acc.gathered += 2 * uint64_t(k);
+    }
+  }
+  __syncthreads();
+  if (tid < 64) {  // exclusive prefix sums of the class sizes, one class per lane
+    const uint32_t c = L.cls[tid];
+    uint32_t x = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (tid >= o) x += y;
+    }
+    L.base[tid] = x - c;
+    if (tid == 63) L.base[kWinClasses] = x;
+  }
+  __syncthreads();
+  if (my_cls < uint32_t(kWinClasses)) L.list[L.base[my_cls] + my_rank] = uint16_t(tid);
+  __syncthreads();
+
+  // work items: the wavefront rows, then rounds of 32 light rows
+  const uint32_t nw = __builtin_amdgcn_readfirstlane(L.cls[0]);
+  const uint32_t nl = __builtin_amdgcn_readfirstlane(L.base[kWinClasses]) - nw;
+  const uint32_t items = nw + (nl + 31) / 32;
+  for (;;) {
+    uint32_t it = 0;
+    if (__lane_id() == 0) it = atomicAdd(&L.q, 1u);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    if (it >= items) break;
+    const bool wave_row = it < nw;
+    const uint32_t pos = wave_row ? it : nw + (it - nw) * 32 + uint32_t(g);
+    if (pos < nw + nl) {
+      const uint32_t p = L.list[pos];
+      const uint32_t v = uint32_t(b0 + p);
+      WinNode wn{};
+      if (gl == 0 && (!wave_row || g == 0)) wn = win_prefetch(a, v);
+      const uint64_t beg = L.off[p], end = L.off[p + 1];
+      const uint64_t own = uint64_t(v) - tlo;
+      uint32_t prev[D], cur[D];
+#pragma unroll
+      for (int b = 0; b < D; b++) prev[b] = cur[b] = L.u.tile[own * CD + gl * D + b];
+      if (wave_row) {
+        win_merge<R, TEST>(a, v, gl, beg + HB_WIN_U * uint64_t(g), end, HB_WIN_U * 32, L.u.tile,
+                           tlo, tlen, cur);
+#pragma unroll
+        for (int o = 2; o < 64; o <<= 1) {
+          uint32_t y[D];
+#pragma unroll
+          for (int b = 0; b < D; b++) y[b] = __shfl_xor(cur[b], o);
+          slab_max<R>(cur, y);
+        }
+      } else {
+        win_merge<R, TEST>(a, v, gl, beg, end, HB_WIN_U, L.u.tile, tlo, tlen, cur);
+      }
+      const uint64_t ob = uint64_t(v) - wlo;
+      const bool was_mod = (L.tmod[ob >> 5] >> (ob & 31)) & 1u;
+      if (!wave_row || g == 0) win_finish<R>(a, v, gl, was_mod, prev, cur, wn, acc);
+    }
+  }
+  __syncthreads();
+  if (tid < kWinClasses) L.cls[tid] = 0;
+  if (tid == 0) L.q = 0;
+}
+
 template <int R, bool TEST>
 __global__ __launch_bounds__(kWinNT) void window_kernel(UnionArgs a) {
   using LT = WinLds<R>;
-  constexpr int D = R, CD = 2 * R, T = LT::kTile;
-  constexpr uint32_t CS = counter_stride(CD);
-  static_assert(T >= kWinB + 64, "the tile holds the block");
+  static_assert(LT::kTile >= kWinB + 64, "the tile holds the block");
   static_assert(sizeof(typename LT::U) >= sizeof(uint32_t) * (kWinNT * (R + 2 * ids_per_lane<2>()) + 2),
                 "the heavy rows' buffers fit the tile");
+  static_assert(kWinClasses == 64 && kWinNT >= 64, "one class per lane of the first wavefront");
   __shared__ LT L;
   Acc acc;
-  const int tid = threadIdx.x, g = __lane_id() / 2, gl = __lane_id() & 1;
+  const int tid = threadIdx.x;
   if (blockIdx.x == 0 && tid == 0) a.stats->list_len = a.count;
 
   // rows longer than heavy_degree: the whole workgroup each
@@ -1285,140 +1433,23 @@ __global__ __launch_bounds__(kWinNT) void window_kernel(UnionArgs a) {
       const uint64_t h = L.item;
       __syncthreads();
       if (h >= a.heavy_count) break;
-      heavy_node<R, 2, 1, kWinNT, false>(a, a.heavy_list[h], L.u.heavy.acc, L.u.heavy.ids,
-                                         L.u.heavy.sum, acc);
+      win_heavy_row<R>(a, h, L, acc);
     }
   }
 
-  const uint64_t n = a.count;
-  const uint32_t B = a.win_block && a.win_block < uint32_t(kWinB) ? a.win_block : uint32_t(kWinB);
-  const uint32_t hmax = uint32_t(T - B) / 2;
-  const uint32_t half = a.win_half && a.win_half < hmax ? a.win_half : hmax;
-  const uint64_t blocks = (n + B - 1) / B;
+  const WinGeom geo = win_geom<R>(a);
   const uint64_t S = gridDim.x / 8;
-  static_assert(kWinClasses == 64 && kWinNT >= 64, "one class per lane of the first wavefront");
   if (tid < kWinClasses) L.cls[tid] = 0;
   if (tid == 0) {
-    L.item = first_item(a.work, blocks, S, blockIdx.x / 8);
+    L.item = first_item(a.work, geo.blocks, S, blockIdx.x / 8);
     L.q = 0;
   }
   __syncthreads();
   for (;;) {
     const uint64_t blk = uniform_u64(L.item);
     if (blk == UINT64_MAX) break;
-    const uint64_t b0 = blk * B;
-    const uint32_t bn = uint32_t(n - b0 < B ? n - b0 : B);
-    const uint64_t tlo = b0 > half ? b0 - half : 0;
-    const uint64_t thi = b0 + bn + half < n ? b0 + bn + half : n;
-    const uint32_t tlen = uint32_t(thi - tlo);
-    const uint64_t wlo = tlo & ~uint64_t(31);
-
-    // stage the tile (64-bit loads, all issued before the stores), its
-    // modified bits and the block's offsets; rank the block's nodes by class
-    {
-      constexpr int H = CD / 2;  // 64-bit words per counter
-      constexpr int PER = (T * H + kWinNT - 1) / kWinNT;
-      const uint2 *src = reinterpret_cast<const uint2 *>(a.curr);
-      uint2 *dst = reinterpret_cast<uint2 *>(L.u.tile);
-      const uint32_t words = tlen * uint32_t(H);
-      constexpr int STEP = 8;  // loads in flight per thread
-#pragma unroll 1
-      for (int j0 = 0; j0 < PER; j0 += STEP) {
-        uint2 w[STEP];
-#pragma unroll
-        for (int j = 0; j < STEP; j++) {
-          const uint32_t k = uint32_t((j0 + j) * kWinNT + tid);
-          w[j] = k < words ? src[(tlo + k / H) * (CS / 2) + k % H] : make_uint2(0u, 0u);
-        }
-#pragma unroll
-        for (int j = 0; j < STEP; j++) {
-          const uint32_t k = uint32_t((j0 + j) * kWinNT + tid);
-          if (k < words) dst[k] = w[j];
-        }
-      }
-    }
-    {
-      const uint64_t w1 = (thi - 1) >> 5;
-      for (uint64_t k = tid; k <= w1 - (wlo >> 5); k += kWinNT) L.tmod[k] = a.curr_mod[(wlo >> 5) + k];
-    }
-    for (uint32_t k = tid; k <= bn; k += kWinNT) L.off[k] = a.off[b0 + k];
-    uint32_t my_cls = kWinClasses, my_rank = 0;
-    if (uint32_t(tid) < bn) {
-      const uint64_t v = b0 + tid;
-      const uint64_t deg = a.off[v + 1] - a.off[v];
-      if (deg <= a.heavy_degree) {
-        my_cls = deg > uint64_t(kWinMedium) ? 0u : uint32_t(kWinMedium + 1 - deg);
-        my_rank = atomicAdd(&L.cls[my_cls], 1u);
-        acc.scanned += deg;
-        acc.checked++;
-      }
-      if (test_bit(a.curr_mod, uint32_t(v))) {
-        const uint32_t k = a.indeg[v];
-        acc.visited += k;
-        acc.gathered += 2 * uint64_t(k);
-      }
-    }
-    __syncthreads();
-    if (tid < 64) {  // exclusive prefix sums of the class sizes, one class per lane
-      const uint32_t c = L.cls[tid];
-      uint32_t x = c;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (tid >= o) x += y;
-      }
-      L.base[tid] = x - c;
-      if (tid == 63) L.base[kWinClasses] = x;
-    }
-    __syncthreads();
-    if (my_cls < uint32_t(kWinClasses)) L.list[L.base[my_cls] + my_rank] = uint16_t(tid);
-    __syncthreads();
-
-    // work items: the wavefront rows, then rounds of 32 light rows
-    const uint32_t nw = __builtin_amdgcn_readfirstlane(L.cls[0]);
-    const uint32_t nl = __builtin_amdgcn_readfirstlane(L.base[kWinClasses]) - nw;
-    const uint32_t items = nw + (nl + 31) / 32;
-    for (;;) {
-      uint32_t it = 0;
-      if (__lane_id() == 0) it = atomicAdd(&L.q, 1u);
-      it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-      if (it >= items) break;
-      const bool wave_row = it < nw;
-      const uint32_t pos = wave_row ? it : nw + (it - nw) * 32 + uint32_t(g);
-      if (pos < nw + nl) {
-        const uint32_t p = L.list[pos];
-        const uint32_t v = uint32_t(b0 + p);
-        WinNode wn{};
-        if (gl == 0 && (!wave_row || g == 0)) wn = win_prefetch(a, v);
-        const uint64_t beg = L.off[p], end = L.off[p + 1];
-        const uint64_t own = uint64_t(v) - tlo;
-        uint32_t prev[D], cur[D];
-#pragma unroll
-        for (int b = 0; b < D; b++) prev[b] = cur[b] = L.u.tile[own * CD + gl * D + b];
-        if (wave_row) {
-          win_merge<R, TEST>(a, v, gl, beg + HB_WIN_U * uint64_t(g), end, HB_WIN_U * 32, L.u.tile,
-                             tlo, tlen, cur);
-#pragma unroll
-          for (int o = 2; o < 64; o <<= 1) {
-            uint32_t y[D];
-#pragma unroll
-            for (int b = 0; b < D; b++) y[b] = __shfl_xor(cur[b], o);
-            slab_max<R>(cur, y);
-          }
-        } else {
-          win_merge<R, TEST>(a, v, gl, beg, end, HB_WIN_U, L.u.tile, tlo, tlen, cur);
-        }
-        const uint64_t ob = uint64_t(v) - wlo;
-        const bool was_mod = (L.tmod[ob >> 5] >> (ob & 31)) & 1u;
-        if (!wave_row || g == 0) win_finish<R>(a, v, gl, was_mod, prev, cur, wn, acc);
-      }
-    }
-    __syncthreads();
-    if (tid < kWinClasses) L.cls[tid] = 0;
-    if (tid == 0) {
-      L.item = grab_item(a.work, blocks, S);
-      L.q = 0;
-    }
+    win_block<R, TEST>(a, blk, geo, L, acc);
+    if (tid == 0) L.item = grab_item(a.work, geo.blocks, S);
     __syncthreads();
   }
   flush_stats(acc, a.stats, a.stat_slots);
@@ -1859,29 +1890,39 @@ struct ShapeWaves {
 // has its own chunk space).
 __global__ void batch_plan_kernel(BatchDesc *d, const uint32_t *shape_inst,
                                   const uint32_t *shape_off, Stats *stats, uint64_t *total_chunks,
-                                  ShapeWaves waves) {
+                                  uint64_t *total_window, ShapeWaves waves) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   for (int sh = 0; sh < kBatchShapes; sh++) {
     uint64_t total = 0;
     for (uint32_t j = shape_off[sh]; j < shape_off[sh + 1]; j++) {
       const BatchDesc &x = d[shape_inst[j]];
-      if (x.active) total += x.list_mode ? uint64_t(*x.u.list_len) : x.n;
+      if (x.active && !x.window) total += x.list_mode ? uint64_t(*x.u.list_len) : x.n;
     }
     const uint32_t npw = shape_nodes_per_wave(uint32_t(sh));
     const uint32_t rounds = chunk_rounds(total, npw, waves.w[sh]);
     const uint64_t ch = uint64_t(rounds) * npw;
-    uint64_t acc = 0;
+    uint64_t acc = 0, wacc = 0, hacc = 0;
     for (uint32_t j = shape_off[sh]; j < shape_off[sh + 1]; j++) {
       BatchDesc &x = d[shape_inst[j]];
       uint64_t count = 0;
       if (x.active) count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
       stats[shape_inst[j]].list_len = count;
+      const bool win = x.active && x.window;
       x.rounds = rounds;
       x.chunk_base = acc;
-      acc += (count + ch - 1) / ch;
+      acc += win ? 0 : (count + ch - 1) / ch;
       x.chunk_end = acc;
+      const uint64_t B = x.u.win_block && x.u.win_block < uint32_t(kWinB) ? x.u.win_block : kWinB;
+      x.wblock_base = wacc;
+      wacc += win ? (count + B - 1) / B : 0;
+      x.wblock_end = wacc;
+      x.wheavy_base = hacc;
+      hacc += win ? x.u.heavy_count : 0;
+      x.wheavy_end = hacc;
     }
     total_chunks[sh] = acc;
+    total_window[2 * sh] = wacc;
+    total_window[2 * sh + 1] = hacc;
   }
 }
 
@@ -1908,7 +1949,7 @@ void batch_union_kernel(const BatchDesc *d, const uint32_t *inst, int cnt,
     if (h >= total_heavy) break;
     const uint32_t i = find_in_shape(d, inst, cnt, h, true);
     const BatchDesc &x = d[i];
-    if (!x.active) continue;
+    if (!x.active || x.window) continue;
     if (int64_t(i) != current) {
       if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
       acc = Acc();
@@ -1934,6 +1975,78 @@ void batch_union_kernel(const BatchDesc *d, const uint32_t *inst, int cnt,
     const uint64_t count = x.list_mode ? uint64_t(*x.u.list_len) : x.n;
     run_chunk<R, G, SPL>(x.u, c - x.chunk_base, count, x.rounds, wave_ids, meta, acc);
     if (__lane_id() == 0) c = grab_chunk(work, chunks);
+  }
+  if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
+}
+
+// Instance whose window block range (or window heavy-row range) contains x,
+// among the instances of one shape.
+__device__ __forceinline__ uint32_t find_window(const BatchDesc *d, const uint32_t *inst, int cnt,
+                                                uint64_t x, bool heavy) {
+  int lo = 0, hi = cnt - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    const uint64_t e = heavy ? d[inst[mid]].wheavy_end : d[inst[mid]].wblock_end;
+    if (e > x) hi = mid;
+    else lo = mid + 1;
+  }
+  return inst[lo];
+}
+
+// The window kernel for the instances of one 64-register shape whose
+// iteration runs as a window iteration (BatchDesc::window): first their rows
+// longer than the window threshold, a workgroup each, then their blocks, from
+// the shape's window spaces (prefix sums over its instances, written by the
+// plan step), dealt per XCD.  The other instances have empty ranges.
+template <int R>
+__global__ __launch_bounds__(kWinNT) void batch_window_kernel(const BatchDesc *d,
+                                                              const uint32_t *inst, int cnt,
+                                                              const uint64_t *total_window,
+                                                              uint32_t *work) {
+  __shared__ WinLds<R> L;
+  Acc acc;
+  int64_t current = -1;
+  const int tid = threadIdx.x;
+  const uint64_t heavy = total_window[1];
+  while (heavy) {
+    if (tid == 0) L.item = atomicAdd(work + 8 * kWorkStride, 1u);
+    __syncthreads();
+    const uint64_t h = L.item;
+    __syncthreads();
+    if (h >= heavy) break;
+    const uint32_t i = find_window(d, inst, cnt, h, true);
+    if (int64_t(i) != current) {
+      if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
+      acc = Acc();
+      current = i;
+    }
+    win_heavy_row<R>(d[i].u, h - d[i].wheavy_base, L, acc);
+  }
+  const uint64_t blocks = total_window[0];
+  const uint64_t S = gridDim.x / 8;
+  if (tid < kWinClasses) L.cls[tid] = 0;
+  if (tid == 0) {
+    L.item = first_item(work, blocks, S, blockIdx.x / 8);
+    L.q = 0;
+  }
+  __syncthreads();
+  for (;;) {
+    const uint64_t blk = uniform_u64(L.item);
+    if (blk == UINT64_MAX) break;
+    const uint32_t i = __builtin_amdgcn_readfirstlane(find_window(d, inst, cnt, blk, false));
+    if (int64_t(i) != current) {
+      if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
+      acc = Acc();
+      current = i;
+    }
+    const BatchDesc &x = d[i];
+    const WinGeom geo = win_geom<R>(x.u);
+    if (x.u.flags & F_UNTESTED)
+      win_block<R, false>(x.u, blk - x.wblock_base, geo, L, acc);
+    else
+      win_block<R, true>(x.u, blk - x.wblock_base, geo, L, acc);
+    if (tid == 0) L.item = grab_item(work, blocks, S);
+    __syncthreads();
   }
   if (current >= 0) flush_stats(acc, d[current].u.stats, d[current].u.stat_slots);
 }
@@ -2266,6 +2379,32 @@ static uint32_t batch_union_grid(int sh, uint32_t grid) {
   return cap && grid > cap ? cap : grid;
 }
 
+// Workgroups of a batched window kernel: the resident ones, a multiple of 8.
+static uint32_t batch_window_grid(int r) {
+  static std::atomic<uint32_t> cache[2][kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  std::atomic<uint32_t> &c = cache[r - 5][dev];
+  uint32_t g = c.load(std::memory_order_relaxed);
+  if (g) return g;
+  int per_cu = 0, cus = 0;
+  const hipError_t e = r == 5 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                    &per_cu, batch_window_kernel<5>, kWinNT, 0)
+                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                    &per_cu, batch_window_kernel<6>, kWinNT, 0);
+  if (e != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  g = std::max<uint32_t>(8, uint32_t(std::max(1, per_cu) * std::max(1, cus)) / 8 * 8);
+  c.store(g, std::memory_order_relaxed);
+  return g;
+}
+
 hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
   ShapeWaves waves{};
   uint32_t grid[kBatchShapes];
@@ -2281,7 +2420,7 @@ hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
     if (marking) batch_mark_kernel<<<dim3(64, L.k), kBlock, 0, s>>>(L.d);
   }
   batch_plan_kernel<<<1, 64, 0, s>>>(L.d, L.shape_inst, L.shape_off, L.stats, L.total_chunks,
-                                     waves);
+                                     L.total_window, waves);
   for (int sh = 0; sh < kBatchShapes; sh++) {
     if (!L.count[sh]) continue;
     const uint32_t *inst = L.shape_inst + L.first[sh];
@@ -2297,6 +2436,13 @@ hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s) {
       HB_BATCH_SHAPES(X)
 #undef X
     }
+    // 64-register shapes: the instances in a window iteration
+    uint32_t *wwork = L.work + (kBatchShapes + size_t(sh)) * kWorkCounters * kWorkStride;
+    const uint64_t *tw = L.total_window + 2 * sh;
+    if (sh == batch_shape(5, 6))
+      batch_window_kernel<5><<<batch_window_grid(5), kWinNT, 0, s>>>(L.d, inst, cnt, tw, wwork);
+    else if (sh == batch_shape(6, 6))
+      batch_window_kernel<6><<<batch_window_grid(6), kWinNT, 0, s>>>(L.d, inst, cnt, tw, wwork);
   }
   batch_reduce_stats_kernel<<<L.k, kStatSlots, 0, s>>>(L.d);
   return hipGetLastError();

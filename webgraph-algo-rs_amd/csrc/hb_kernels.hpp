@@ -179,10 +179,18 @@ struct BatchDesc {
   uint64_t heavy_base, heavy_end;  // in the shape's heavy-row space
   uint32_t rounds;                 // rounds per chunk of the shape, written by the plan step
   uint32_t mark_first;             // mark the predecessors of the modified nodes into must first
+  // This iteration runs as the window kernel (64-register counters, every
+  // node checked; u carries its settings): the instance takes no chunks and
+  // no heavy rows of batch_union_kernel, but blocks and heavy rows of the
+  // shape's window spaces, written by the plan step.
+  uint32_t window;
+  uint64_t wblock_base, wblock_end;
+  uint64_t wheavy_base, wheavy_end;
 };
 
-// Queue counters of a batch: one region of kWorkCounters per shape.
-constexpr int kBatchWorkWords = kBatchShapes * kWorkCounters * kWorkStride;
+// Queue counters of a batch: one region of kWorkCounters per shape for the
+// union kernels, then one per shape for the window kernels.
+constexpr int kBatchWorkWords = 2 * kBatchShapes * kWorkCounters * kWorkStride;
 
 // One batched iteration.  Instances are grouped by shape (`shape_inst`,
 // `shape_off` on the device; `first`/`count` on the host): each shape has its
@@ -194,6 +202,7 @@ struct BatchLaunch {
   Stats *stats;
   uint32_t *work;           // kBatchWorkWords
   uint64_t *total_chunks;   // kBatchShapes, written by the plan step
+  uint64_t *total_window;   // 2 * kBatchShapes (blocks, heavy rows), written by the plan step
   const uint32_t *shape_inst;
   const uint32_t *shape_off;  // kBatchShapes + 1
   uint32_t first[kBatchShapes], count[kBatchShapes];
@@ -203,7 +212,7 @@ struct BatchLaunch {
 
 // Launches one batched iteration on s: clear the next bitmaps and statistics,
 // compact the worklists of list-mode instances, plan the chunk spaces, then
-// per shape the heavy-row and union kernels.
+// per shape the union kernel and, for 64-register shapes, the window kernel.
 hipError_t launch_batch_iteration(const BatchLaunch &L, hipStream_t s);
 
 // Chooses the lane group of a (r, log2m) shape: lanes per node, slabs per
