@@ -129,6 +129,7 @@ def cpu_baseline(orc, hb, g, gt, args, log2m, reg_bits):
 
 
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "union_step_web1e7_log2m10_r6_pmc.md")
+SMALL_TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "union_step_web1e7_log2m6_r5.md")
 KERNEL_SOURCE = os.path.join(PKG, "csrc", "hb_kernels.hip")
 
 
@@ -138,15 +139,15 @@ def kernel_source_sha():
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def profiled_traffic(args):
+def profiled_traffic(args, profile=TRAFFIC_PROFILE, config=(10_000_000, 10, 6)):
     """HBM bytes per union step from the committed FETCH_SIZE / WRITE_SIZE
     passes (copy-calibrated), for the configuration they were taken on, and
     only when they were taken on the kernel source that is built now."""
-    if (args.nodes, args.log2m, args.reg_bits) != (10_000_000, 10, 6):
+    if (args.nodes, args.log2m, args.reg_bits) != config:
         return None, "no profile for this configuration"
     value, sha = None, None
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(profile) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_union_step:"):
                     value = float(line.split(":")[1])
@@ -156,7 +157,7 @@ def profiled_traffic(args):
         return None, "profile missing"
     if sha != kernel_source_sha():
         return None, "the profile was taken on other kernel sources"
-    return value, os.path.relpath(TRAFFIC_PROFILE, REPO) + f" (hb_kernels.hip sha256 {sha[:16]})"
+    return value, os.path.relpath(profile, REPO) + f" (hb_kernels.hip sha256 {sha[:16]})"
 
 
 def batch_workload(hbm, args):
@@ -344,6 +345,54 @@ def bench_precision(hbm, args):
                          "f64 engine in the parity tests)", "modes": out}
 
 
+def bench_small_counters(hbm, args):
+    """The headline graph (n = 10^7, the same generator and seeds) at log2m 6
+    with 5-bit registers: 40-byte counters, whose iterations that check every
+    node run as the window kernel (counters around each block of consecutive
+    nodes staged in LDS).  One warmup run, then args.steps timed runs to
+    stabilisation; the union step's roofline uses the headline's model with
+    C = 40 bytes, and the traffic comes from the committed profile of this
+    configuration when it matches the kernel source."""
+    g = hbm.Graph.web(args.nodes, 2.1, 3, 10000, 1000, 0.7, 2)
+    gt = g.transpose()
+    hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 6).register_bits(5).jenkins(42)
+          .sum_of_distances(True).sum_of_inverse_distances(True).build())
+    hb.set_tuning(512)
+    hb.set_timing(True)
+    hb.run_until_done()
+    work, union_ms, nbytes, launches, windows, iters = 0, 0.0, 0.0, 0, 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hb.run_until_done()
+        st = hb.iteration_stats()
+        iters = len(st)
+        for s_ in st:
+            work += s_["visited_arcs"] * 64
+            union_ms += s_["union_ms"]
+            nbytes += algorithmic_bytes(s_, g.num_nodes, 40, 4 * 5, 2)
+            launches += 1
+            windows += s_["window"]
+    elapsed = time.perf_counter() - t0
+    hb.close()
+    achieved = nbytes / (union_ms * 1e-3) / 1e9 if union_ms > 0 else 0.0
+    small = argparse.Namespace(nodes=args.nodes, log2m=6, reg_bits=5)
+    traffic, src = profiled_traffic(small, SMALL_TRAFFIC_PROFILE, (10_000_000, 6, 5))
+    per_launch = nbytes / max(1, launches)
+    print(f"# small counters (log2m 6): {elapsed * 1e3 / args.steps:.1f} ms per run", file=sys.stderr,
+          flush=True)
+    return {"workload": f"web graph n={g.num_nodes} arcs={g.num_arcs}, log2m 6, r 5, jenkins(42), "
+                        "systolic frontier, run to stabilisation",
+            "value": work / elapsed, "unit": "arc-registers/s",
+            "ms_per_step": elapsed * 1e3 / args.steps, "iterations": iters,
+            "window_iterations": windows // max(1, args.steps),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": achieved / PEAK_HBM_GBS, "avg_launch_ms": union_ms / max(1, launches),
+                         "algorithmic_bytes_per_launch": per_launch, "traffic": traffic,
+                         "traffic_over_algorithmic": traffic / per_launch if traffic else None,
+                         "traffic_source": src, "kernel": "union step (window_kernel in the "
+                         "iterations that check every node, union_kernel in the others)"}}
+
+
 def bench_host_counters(hbm, args):
     """Counter arrays in HBM against pinned host memory, read and written by
     the kernels over the host link (COUNTERS_HOST), and against host memory
@@ -482,6 +531,8 @@ def main():
     ap.add_argument("--large-nodes", type=int, default=100_000_000)
     ap.add_argument("--large-steps", type=int, default=3, help="timed runs of the large instance")
     ap.add_argument("--no-large", action="store_true", help="skip the large instance leg")
+    ap.add_argument("--no-small-counters", action="store_true",
+                    help="skip the log2m-6 leg on the headline graph")
     ap.add_argument("--no-precision", action="store_true", help="skip the precision leg")
     ap.add_argument("--host-counters", action="store_true",
                     help="only the counters-in-host-memory comparison (n = --host-nodes)")
@@ -624,6 +675,8 @@ def main():
         result["cpu_baseline"] = cpu_baseline(orc, hb, g, gt, args, args.log2m, args.reg_bits)
     hb.close()
     del g, gt
+    if not args.no_small_counters:
+        result["small_counters"] = bench_small_counters(hbm, args)
     if args.batch:
         result["batched"] = bench_batch(hbm, args)
     if not args.no_precision:

@@ -303,51 +303,6 @@ __device__ __forceinline__ void acc_add(void *base, uint32_t mode, uint32_t v, d
   }
 }
 
-// acc_add in three steps, for callers that read the accumulator early: the
-// raw 8 bytes of node v's accumulator (4 in the f32 mode, in .x), the updated
-// raw value, and its store.  acc_store(acc_apply(acc_load(...), x)) is
-// acc_add.
-__device__ __forceinline__ uint2 acc_load(const void *base, uint32_t mode, uint32_t v) {
-  uint2 r;
-  if (mode == ACC_F32) {
-    r.x = static_cast<const uint32_t *>(base)[v];
-    r.y = 0;
-  } else {
-    r = static_cast<const uint2 *>(base)[v];
-  }
-  return r;
-}
-
-__device__ __forceinline__ uint2 acc_apply(uint2 raw, uint32_t mode, double x) {
-  uint2 r;
-  if (mode == ACC_F64) {
-    const double d =
-        __builtin_bit_cast(double, uint64_t(raw.y) << 32 | uint64_t(raw.x)) + x;
-    const uint64_t b = __builtin_bit_cast(uint64_t, d);
-    r.x = uint32_t(b);
-    r.y = uint32_t(b >> 32);
-  } else if (mode == ACC_F32_KAHAN) {
-    float vx = __builtin_bit_cast(float, raw.x), vy = __builtin_bit_cast(float, raw.y);
-    const float y = float(x) - vy;
-    const float t = vx + y;
-    vy = (t - vx) - y;
-    vx = t;
-    r.x = __builtin_bit_cast(uint32_t, vx);
-    r.y = __builtin_bit_cast(uint32_t, vy);
-  } else {
-    r.x = __builtin_bit_cast(uint32_t, __builtin_bit_cast(float, raw.x) + float(x));
-    r.y = 0;
-  }
-  return r;
-}
-
-__device__ __forceinline__ void acc_store(void *base, uint32_t mode, uint32_t v, uint2 raw) {
-  if (mode == ACC_F32)
-    static_cast<uint32_t *>(base)[v] = raw.x;
-  else
-    static_cast<uint2 *>(base)[v] = raw;
-}
-
 // acc_add onto an accumulator known to hold zero (iteration 0: init zeroes
 // them): the same arithmetic on a zero operand, with no read.
 __device__ __forceinline__ void acc_add_to_zero(void *base, uint32_t mode, uint32_t v, double x) {
