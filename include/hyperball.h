@@ -319,12 +319,14 @@ hb_status hb_set_tuning(hb_instance *hb, uint32_t heavy_degree);
  * share of changed slabs is below 0.7; 1: always; 2: never. */
 hb_status hb_set_delta_gathering(hb_instance *hb, int mode);
 /* Window kernel, for 64-register counters (log2m 6) in device memory: in an
- * iteration that checks every node and reads the counters (not the seeded
- * first one), each workgroup takes blocks of consecutive nodes and first
+ * iteration that reads the counters (not the seeded first one, no delta
+ * gathering), each workgroup takes blocks of consecutive nodes and first
  * copies the current counters of the nodes around its block (the tile:
  * half_width nodes on each side) into LDS, where successors inside it are
  * read; the others are read from HBM.  Exact, like every kernel choice.
- * mode 0 (default) and 1: in those iterations; 2: never.  block_nodes and
+ * mode 0 (default): the iterations that check every node, and worklist
+ * (systolic or local) iterations expected to check at least an eighth of the
+ * nodes; 1: every iteration it can run; 2: never.  block_nodes and
  * half_width cap the kernel's own block and tile (0: the kernel's largest);
  * rows longer than heavy_degree take a workgroup each (0: the larger of
  * hb_set_tuning's threshold and 8192; UINT32_MAX: none), shorter ones a

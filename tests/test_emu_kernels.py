@@ -75,17 +75,23 @@ def test_emulated_engine_matches_oracle(emu_exes, log2m, r, n, dmax, seed, heavy
                                          # rows above 100 on the whole workgroup
     (5, 2500, 400, 4, "0 128 0 4294967295"),  # the largest tile around 128-node blocks, every
                                               # long row on a wavefront
+    (6, 2000, 600, 11, "1 64 40 100"),   # every iteration it can run: systolic and local
+                                         # worklists too (bitmaps, visited arcs per arc)
 ])
 def test_emulated_window_kernel_matches_oracle(emu_exes, r, n, dmax, seed, window):
-    """The window kernel (log2m 6, iterations that check every node; delta
-    gathering off so that they all take it) against the oracle every
-    iteration, visited arcs included, under AddressSanitizer."""
+    """The window kernel (log2m 6, delta gathering off) against the oracle
+    every iteration, visited arcs included, under AddressSanitizer: the
+    iterations that check every node, and in mode 1 the systolic and local
+    worklist iterations as well."""
     args = [emu_exes["oracle_check"], "6", str(r), str(n), str(dmax), str(seed), "64", "2"]
     env = dict(os.environ, HB_CHECK_WINDOW=window)
     res = subprocess.run(args, capture_output=True, text=True, timeout=900, env=env)
     assert res.returncode == 0 and res.stdout.strip().endswith("OK"), res.stdout[-3000:] + res.stderr[-6000:]
     assert "DIFFERENT" not in res.stdout
     assert res.stdout.count(" window 1 ") >= 5
+    if window.startswith("1"):
+        modes = re.findall(r"mode (\d+) .* window 1 ", res.stdout)
+        assert {"1", "5", "7"} <= set(modes), modes
 
 
 def test_emulated_count_high_registers_match_oracle(emu_exes):

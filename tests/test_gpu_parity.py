@@ -143,9 +143,14 @@ def test_cnr_2000_par_equals_seq_every_iteration(hbm, orc, cnr):
     it, stats = _run_lockstep(hbm, orc, g, gt, 6)
     assert it > 10
     assert any(s["systolic"] for s in stats) and any(s["local"] for s in stats)
-    # the first systolic iteration ran over the marked predecessors, not every node
+    # the first systolic iteration ran over the marked predecessors, or, on the
+    # window kernel, checked every node as the reference does
     first = next(i for i, s in enumerate(stats) if s["systolic"])
-    assert not stats[first - 1]["systolic"] and stats[first]["checked"] < g.num_nodes
+    assert not stats[first - 1]["systolic"]
+    if stats[first]["window"]:
+        assert stats[first]["checked"] == g.num_nodes
+    else:
+        assert stats[first]["checked"] < g.num_nodes
 
 
 @pytest.mark.parametrize("heavy", [64, 0, 2**32 - 1])
@@ -201,32 +206,35 @@ def test_window_kernel_bit_exact(hbm, orc, window, reg_bits):
     assert [s["window"] for s in stats] == [int(i > 0 and window[0] != 2) for i in range(it)]
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("reg_bits", [5, 6])
 @pytest.mark.parametrize("acc", ["ACC_F64", "ACC_F32_KAHAN", "ACC_F32"])
-def test_window_kernel_engine_outputs(hbm, orc, reg_bits, acc):
+def test_window_kernel_engine_outputs(hbm, orc, reg_bits, acc, mode):
     """Window iterations against the oracle's restatement of the reference
-    engine, with a transpose (dense iterations, then systolic ones on the
-    union kernel): registers, modified bits and the visited arcs the kernel
-    counts from in-degrees, every iteration; sums of distances and harmonic
-    centralities bit-exact in f64 and within 1e-6 relative with f32
-    accumulators (Kahan-compensated or not)."""
+    engine, with a transpose (dense iterations, then systolic and local
+    worklists: large ones on the window kernel by the rule in mode 0, all of
+    them in mode 1): registers, modified bits, visited arcs (from in-degrees
+    or counted per arc) and modified counts every iteration; sums of
+    distances and harmonic centralities bit-exact in f64 and within 1e-6
+    relative with f32 accumulators (Kahan-compensated or not)."""
     g = hbm.Graph.web(30000, 2.1, 1, 3000, 400, 0.7, 21)
     gt = g.transpose()
     hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, gt, 6).register_bits(reg_bits).jenkins(11)
           .sum_of_distances(True).sum_of_inverse_distances(True)
           .accumulator_precision(getattr(hbm, acc)).build())
-    hb.set_window(0, 512, 300, 200)
+    hb.set_window(mode, 512, 300, 200)
     ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=6,
                               reg_bits=reg_bits, hash=orc.HASH_JENKINS, seed=11,
                               sum_of_distances=True, sum_of_inverse_distances=True)
     hb.init()
     ref.init()
-    windows = 0
+    windows, systolic_windows = 0, 0
     for it in range(1000):
         hb.iterate()
         ref.iterate()
         s = hb.iteration_stats()[-1]
         windows += s["window"]
+        systolic_windows += s["window"] and s["systolic"]
         assert np.array_equal(hb.counters(0), ref.counters(0)), f"registers differ at {it}"
         assert np.array_equal(hb.modified_bits(), ref.modified_bits()), f"modified bits differ at {it}"
         assert s["visited_arcs"] == ref.visited_arcs, f"visited arcs differ at {it}"
@@ -234,6 +242,7 @@ def test_window_kernel_engine_outputs(hbm, orc, reg_bits, acc):
         if ref.modified == 0:
             break
     assert windows >= 3
+    assert systolic_windows >= 1
     if acc == "ACC_F64":
         assert np.array_equal(hb.sum_of_distances(), ref.sum_of_distances())
         assert np.array_equal(hb.harmonic_centralities(), ref.harmonic_centralities())

@@ -135,17 +135,21 @@ struct Instance {
   bool delta = false;         // the last planned iteration gathers deltas
   int delta_mode = 0;         // 0: by the rule above, 1: always, 2: never
   // Window kernel (launch_window_union): the union step of iterations that
-  // check every node and read the counters, for 64-register counters in
-  // device memory.  indeg[s] counts the arcs into s from other nodes (the
-  // kernel counts visited arcs from it).  Rows longer than kWindowHeavy (or
-  // the heavy-row threshold, if larger) take a workgroup each; shorter ones a
-  // wavefront or a lane group inside their block.  Successors outside the
-  // tile are read untested while at least kUntestedShare of the nodes changed
-  // in the last iteration.
+  // read the counters (not the seeded first one, no delta gathering), for
+  // 64-register counters in device memory: those that check every node, and
+  // worklist iterations (systolic or local) whose worklist is expected to
+  // hold at least kWindowListShare of the nodes (they need no compacted
+  // list: the kernel reads the bitmaps).  indeg[s] counts the arcs into s
+  // from other nodes (iterations that check every node count visited arcs
+  // from it).  Rows longer than kWindowHeavy (or the heavy-row threshold, if
+  // larger) take a workgroup each; shorter ones a wavefront or a lane group
+  // inside their block.  Successors outside the tile are read untested while
+  // at least kUntestedShare of the nodes changed in the last iteration.
   static constexpr uint64_t kWindowHeavy = 8192;
   static constexpr double kUntestedShare = 0.9;
+  static constexpr double kWindowListShare = 0.125;
   DevBuf<uint32_t> indeg;
-  int window_mode = 0;  // 0: by the rule above, 1: the same, 2: never
+  int window_mode = 0;  // 0: by the rule above, 1: whenever the kernel can run, 2: never
   uint32_t win_block = 0, win_half = 0;
   uint64_t win_heavy = 0;  // 0: max(heavy_degree, kWindowHeavy)
   bool window_now = false;  // the iteration under way runs the window kernel
@@ -606,10 +610,28 @@ struct Instance {
   // heavy-row threshold (a prefix of heavy_nodes, sorted by decreasing degree).
   bool plan_window(Plan &p, uint64_t modified_prev) {
     const uint32_t f = p.ua.flags;
-    if (window_mode == 2 || !indeg.p || p.list_mode || !(f & F_CHECK_ALL) ||
-        (f & (F_LIST | F_SEEDED | F_DELTA | F_SYSTOLIC)))
+    if (window_mode == 2 || !indeg.p || (f & (F_SEEDED | F_DELTA))) return false;
+    if (p.list_mode || (f & F_SYSTOLIC)) {
+      // A worklist iteration runs as a window iteration when its worklist is
+      // expected to hold a large share of the nodes: the modified nodes and
+      // their predecessors (about the mean in-degree each).
+      const double expected = double(modified_prev) * (1.0 + double(a) / double(n));
+      if (window_mode == 0 && expected < kWindowListShare * double(n)) return false;
+      if (p.mark_first) {
+        // The first systolic iteration checks every node, as the reference
+        // does (no predecessors were marked), instead of marking the
+        // predecessors of the modified nodes first: a node with no modified
+        // successor merges nothing, so the counters are the same, and the
+        // kernel tests each successor's modified bit anyway.
+        p.mark_first = false;
+        p.list_mode = false;
+        p.ua.flags = (p.ua.flags & ~F_LIST) | F_CHECK_ALL;
+      }
+    } else if (!(f & F_CHECK_ALL)) {
       return false;
-    if (double(modified_prev) >= kUntestedShare * double(n)) p.ua.flags |= F_UNTESTED;
+    } else if (double(modified_prev) >= kUntestedShare * double(n)) {
+      p.ua.flags |= F_UNTESTED;
+    }
     const uint64_t h = win_heavy ? win_heavy : std::max(heavy_degree, kWindowHeavy);
     uint64_t k = 0;
     while (k < heavy_degs.size() && heavy_degs[k] > h) k++;
@@ -640,7 +662,7 @@ struct Instance {
       HB_HIP(launch_mark_predecessors(list.p, list_len.p, toff.p, tsucc.p, must[mustcur].p,
                                       max_grid, stream));
     }
-    if (p.list_mode)
+    if (p.list_mode && !window_now)
       HB_HIP(launch_compact(p.ua.must, p.ua.curr_mod, n, bsum.p, list.p, list_len.p, stream));
     if (timing) HB_HIP(hipEventRecord(ev[1], stream));
     if (window_now)
@@ -905,7 +927,7 @@ struct Batch {
       d.u.work = work.p;
       d.u.stat_slots = slots.p + i * size_t(kStatSlots) * kStatSlotWords;
       d.active = 1;
-      d.list_mode = plans[i].list_mode;
+      d.list_mode = plans[i].list_mode && !d.window;  // a window iteration reads the bitmaps
       d.mark_first = plans[i].mark_first;
       d.clear_must = inst[i]->systolic;
     }

@@ -1100,14 +1100,19 @@ struct WinLds {
 // itself (a self loop, or a slot past the row end), which changes nothing;
 // with TEST an outside successor is read only when its modified bit is set,
 // which saves its counter read in iterations where few nodes changed.  The
-// tile holds v (its block lies inside the tile).
+// tile holds v (its block lies inside the tile).  With COUNT, nvisit counts
+// the modified successors other than v (the visited arcs of
+// hyperball_impl.rs:1038-1039), from the tile's modified bits (tmod, from the
+// word of tlo on) inside the tile.
 #ifndef HB_WIN_U
 #define HB_WIN_U 4
 #endif
-template <int R, bool TEST>
+template <int R, bool TEST, bool COUNT = false>
 __device__ __forceinline__ void win_merge(const UnionArgs &a, uint32_t v, int gl, uint64_t first,
                                           uint64_t end, uint64_t stride, const uint32_t *tile,
-                                          uint64_t tlo, uint32_t tlen, uint32_t *cur) {
+                                          const uint32_t *tmod, uint64_t tlo, uint32_t tlen,
+                                          uint32_t *cur, uint32_t &nvisit) {
+  static_assert(TEST || !COUNT, "counting visited arcs needs the modified bits");
   constexpr int D = R, CD = 2 * R, WU = HB_WIN_U;  // loads in flight per lane
   constexpr uint32_t CS = counter_stride(CD);
   const uint32_t *gbase = a.curr + gl * D;
@@ -1136,10 +1141,17 @@ __device__ __forceinline__ void win_merge(const UnionArgs &a, uint32_t v, int gl
     for (int u = 0; u < WU; u++) {
       const uint32_t t = ids[u];
       // tlo + tlen <= n <= 2^32, so t - tlo wraps to at least tlen when t < tlo
-      const bool out = t - uint32_t(tlo) >= tlen;
+      const uint32_t ti = t - uint32_t(tlo);
+      const bool out = ti >= tlen;
+      bool mod = true;
+      if (TEST) {
+        const uint32_t b = ti + uint32_t(tlo & 31u);  // bit of t in tmod (tlo's word first)
+        mod = out ? test_bit(a.curr_mod, t) : ((tmod[b >> 5] >> (b & 31)) & 1u);
+      }
+      if (COUNT) nvisit += (mod && t != v) ? 1u : 0u;
 #pragma unroll
       for (int b = 0; b < D; b++) yg[u][b] = 0u;
-      if (out && (!TEST || test_bit(a.curr_mod, t))) load_part<D>(yg[u], gbase + uint64_t(t) * CS);
+      if (out && mod) load_part<D>(yg[u], gbase + uint64_t(t) * CS);
     }
 #pragma unroll
     for (int u = 0; u < WU; u++) {
@@ -1168,16 +1180,20 @@ __device__ __forceinline__ WinNode win_prefetch(const UnionArgs &a, uint32_t v) 
 }
 
 // The end of the node body (hyperball_impl.rs:1051-1144) for a checked node
-// of a window iteration (not systolic, counters read; visited arcs, scanned
-// arcs and checked nodes counted when the block is staged): node_update and
-// the store of finish_node, with the node data of win_prefetch (valid on
-// lane 0 of the group).
-template <int R>
+// of a window iteration: node_update and the store of finish_node, with the
+// node data of win_prefetch (valid on lane 0 of the group).  Without SYS
+// (every node checked, not systolic) visited arcs, scanned arcs and checked
+// nodes are counted when the block is staged; with SYS (a worklist
+// iteration, systolic or local) the node's nvisit modified successors count
+// here, the neighbourhood function takes post - pre in systolic iterations,
+// and a modified node marks its predecessors.
+template <int R, bool SYS>
 __device__ __forceinline__ void win_finish(const UnionArgs &a, uint32_t v, int gl, bool was_mod,
                                            const uint32_t *prev, const uint32_t *cur,
-                                           const WinNode &w, Acc &acc) {
+                                           uint32_t nvisit, const WinNode &w, Acc &acc) {
   constexpr int D = R;
   constexpr uint32_t CS = counter_stride(2 * R);
+  const bool systolic = SYS && (a.flags & F_SYSTOLIC);
   uint32_t d = 0;
 #pragma unroll
   for (int b = 0; b < D; b++) d |= cur[b] ^ prev[b];
@@ -1193,7 +1209,7 @@ __device__ __forceinline__ void win_finish(const UnionArgs &a, uint32_t v, int g
       a.chg_next[v] = chg_bits;
       acc.changed += __popc(chg_bits);
       a.cache[v] = post;
-      acc.add_nf(post);
+      acc.add_nf(systolic ? post - w.pre : post);
       if (a.flags & F_CENTRALITY) {
         const double delta = post - w.pre;
         if (delta > 0.0) {
@@ -1204,9 +1220,21 @@ __device__ __forceinline__ void win_finish(const UnionArgs &a, uint32_t v, int g
       }
       atomicOr(a.next_mod + (v >> 5), 1u << (v & 31));
       acc.modified++;
-    } else {
+    } else if (!systolic) {
       acc.add_nf(w.pre);
     }
+    if (SYS) {
+      acc.visited += nvisit;
+      acc.gathered += 2 * uint64_t(nvisit);
+    }
+  }
+  if (systolic && cm) {
+    const uint64_t tb = a.toff[v], te = a.toff[v + 1];
+    for (uint64_t k = tb + gl; k < te; k += 2) {
+      const uint32_t p = a.tsucc[k];
+      atomicOr(a.next_must + (p >> 5), 1u << (p & 31));
+    }
+    if (gl == 0) acc.tscanned += te - tb;
   }
   if (cm || was_mod) store_part<D>(a.next + uint64_t(v) * CS + gl * D, cur);
 }
@@ -1238,20 +1266,26 @@ __device__ __forceinline__ WinGeom win_geom(const UnionArgs &a) {
 }
 
 // Rows longer than a.heavy_degree: heavy row h of a.heavy_list, the whole
-// workgroup (visited arcs counted at staging, as for every node).
+// workgroup (visited arcs counted at staging, as for every node, except in
+// worklist iterations).
 template <int R>
 __device__ __forceinline__ void win_heavy_row(const UnionArgs &a, uint64_t h, WinLds<R> &L,
                                               Acc &acc) {
-  heavy_node<R, 2, 1, kWinNT, false>(a, a.heavy_list[h], L.u.heavy.acc, L.u.heavy.ids,
-                                     L.u.heavy.sum, acc);
+  if (a.flags & (F_SYSTOLIC | F_LIST))  // worklist iterations count visited arcs per row
+    heavy_node<R, 2, 1, kWinNT, true>(a, a.heavy_list[h], L.u.heavy.acc, L.u.heavy.ids,
+                                      L.u.heavy.sum, acc);
+  else
+    heavy_node<R, 2, 1, kWinNT, false>(a, a.heavy_list[h], L.u.heavy.acc, L.u.heavy.ids,
+                                       L.u.heavy.sum, acc);
 }
 
 // Block blk of the instance a: stage its tile, rank its nodes, work on them.
 // Every thread of the workgroup calls it; L.cls and L.q are zero on entry and
 // on return, and the workgroup has passed a barrier after the work.
-template <int R, bool TEST>
+template <int R, bool TEST, bool SYS>
 __device__ __forceinline__ void win_block(const UnionArgs &a, uint64_t blk, const WinGeom &geo,
                                           WinLds<R> &L, Acc &acc) {
+  static_assert(TEST || !SYS, "worklist iterations test modified bits");
   constexpr int D = R, CD = 2 * R, T = WinLds<R>::kTile;
   constexpr uint32_t CS = counter_stride(CD);
   const int tid = threadIdx.x, g = __lane_id() / 2, gl = __lane_id() & 1;
@@ -1296,13 +1330,29 @@ __device__ __forceinline__ void win_block(const UnionArgs &a, uint64_t blk, cons
   if (uint32_t(tid) < bn) {
     const uint64_t v = b0 + tid;
     const uint64_t deg = a.off[v + 1] - a.off[v];
+    const bool was_mod = test_bit(a.curr_mod, uint32_t(v));
+    // worklist iterations (F_LIST): the listed nodes (must be checked, or
+    // modified), all checked in local iterations (F_CHECK_ALL), and the
+    // unchecked modified ones copied; otherwise every node, checked
+    const bool must = SYS && test_bit(a.must, uint32_t(v));
+    const bool listed = !SYS || !(a.flags & F_LIST) || must || was_mod;
+    const bool check = !SYS || (a.flags & F_CHECK_ALL) || must;
     if (deg <= a.heavy_degree) {
-      my_cls = deg > uint64_t(kWinMedium) ? 0u : uint32_t(kWinMedium + 1 - deg);
-      my_rank = atomicAdd(&L.cls[my_cls], 1u);
-      acc.scanned += deg;
-      acc.checked++;
+      if (listed && check) {
+        my_cls = deg > uint64_t(kWinMedium) ? 0u : uint32_t(kWinMedium + 1 - deg);
+        my_rank = atomicAdd(&L.cls[my_cls], 1u);
+        acc.scanned += deg;
+        acc.checked++;
+      } else if (SYS && was_mod) {
+        constexpr int CDW = 2 * R;
+        const uint32_t *src = a.curr + v * CS;
+        uint32_t *dst = a.next + v * CS;
+#pragma unroll
+        for (int b = 0; b < CDW; b++) dst[b] = src[b];
+        acc.copied++;
+      }
     }
-    if (test_bit(a.curr_mod, uint32_t(v))) {
+    if (!SYS && was_mod) {
       const uint32_t k = a.indeg[v];
       acc.visited += k;
       acc.gathered += 2 * uint64_t(k);
@@ -1345,9 +1395,10 @@ __device__ __forceinline__ void win_block(const UnionArgs &a, uint64_t blk, cons
       uint32_t prev[D], cur[D];
 #pragma unroll
       for (int b = 0; b < D; b++) prev[b] = cur[b] = L.u.tile[own * CD + gl * D + b];
+      uint32_t nvisit = 0;
       if (wave_row) {
-        win_merge<R, TEST>(a, v, gl, beg + HB_WIN_U * uint64_t(g), end, HB_WIN_U * 32, L.u.tile,
-                           tlo, tlen, cur);
+        win_merge<R, TEST, SYS>(a, v, gl, beg + HB_WIN_U * uint64_t(g), end, HB_WIN_U * 32,
+                                L.u.tile, L.tmod, tlo, tlen, cur, nvisit);
 #pragma unroll
         for (int o = 2; o < 64; o <<= 1) {
           uint32_t y[D];
@@ -1355,12 +1406,19 @@ __device__ __forceinline__ void win_block(const UnionArgs &a, uint64_t blk, cons
           for (int b = 0; b < D; b++) y[b] = __shfl_xor(cur[b], o);
           slab_max<R>(cur, y);
         }
+        if (SYS) {
+          nvisit = gl == 0 ? nvisit : 0u;
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) nvisit += __shfl_xor(nvisit, o);
+        }
       } else {
-        win_merge<R, TEST>(a, v, gl, beg, end, HB_WIN_U, L.u.tile, tlo, tlen, cur);
+        win_merge<R, TEST, SYS>(a, v, gl, beg, end, HB_WIN_U, L.u.tile, L.tmod, tlo, tlen, cur,
+                                nvisit);
       }
       const uint64_t ob = uint64_t(v) - wlo;
       const bool was_mod = (L.tmod[ob >> 5] >> (ob & 31)) & 1u;
-      if (!wave_row || g == 0) win_finish<R>(a, v, gl, was_mod, prev, cur, wn, acc);
+      if (!wave_row || g == 0)
+        win_finish<R, SYS>(a, v, gl, was_mod, prev, cur, nvisit, wn, acc);
     }
   }
   __syncthreads();
@@ -1368,7 +1426,7 @@ __device__ __forceinline__ void win_block(const UnionArgs &a, uint64_t blk, cons
   if (tid == 0) L.q = 0;
 }
 
-template <int R, bool TEST>
+template <int R, bool TEST, bool SYS>
 __global__ __launch_bounds__(kWinNT) void window_kernel(UnionArgs a) {
   using LT = WinLds<R>;
   static_assert(LT::kTile >= kWinB + 64, "the tile holds the block");
@@ -1403,7 +1461,7 @@ __global__ __launch_bounds__(kWinNT) void window_kernel(UnionArgs a) {
   for (;;) {
     const uint64_t blk = uniform_u64(L.item);
     if (blk == UINT64_MAX) break;
-    win_block<R, TEST>(a, blk, geo, L, acc);
+    win_block<R, TEST, SYS>(a, blk, geo, L, acc);
     if (tid == 0) L.item = grab_item(a.work, geo.blocks, S);
     __syncthreads();
   }
@@ -1996,10 +2054,12 @@ __global__ __launch_bounds__(kWinNT) void batch_window_kernel(const BatchDesc *d
     }
     const BatchDesc &x = d[i];
     const WinGeom geo = win_geom<R>(x.u);
-    if (x.u.flags & F_UNTESTED)
-      win_block<R, false>(x.u, blk - x.wblock_base, geo, L, acc);
+    if (x.u.flags & (F_SYSTOLIC | F_LIST))
+      win_block<R, true, true>(x.u, blk - x.wblock_base, geo, L, acc);
+    else if (x.u.flags & F_UNTESTED)
+      win_block<R, false, false>(x.u, blk - x.wblock_base, geo, L, acc);
     else
-      win_block<R, true>(x.u, blk - x.wblock_base, geo, L, acc);
+      win_block<R, true, false>(x.u, blk - x.wblock_base, geo, L, acc);
     if (tid == 0) L.item = grab_item(work, blocks, S);
     __syncthreads();
   }
@@ -2268,7 +2328,7 @@ uint32_t window_tile_nodes(int r) {
   return r == 5 ? uint32_t(WinLds<5>::kTile) : r == 6 ? uint32_t(WinLds<6>::kTile) : 0u;
 }
 
-template <int R, bool TEST>
+template <int R, bool TEST, bool SYS>
 static hipError_t launch_window_t(const UnionArgs &a, hipStream_t s) {
   static std::atomic<uint32_t> cache[kMaxDevices];
   uint32_t grid = 0;
@@ -2278,7 +2338,8 @@ static hipError_t launch_window_t(const UnionArgs &a, hipStream_t s) {
       grid = cache[dev].load(std::memory_order_relaxed);
       if (!grid) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, window_kernel<R, TEST>, kWinNT, 0) ==
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, window_kernel<R, TEST, SYS>,
+                                                         kWinNT, 0) ==
                 hipSuccess &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
                 hipSuccess) {
@@ -2294,15 +2355,19 @@ static hipError_t launch_window_t(const UnionArgs &a, hipStream_t s) {
   uint64_t g = grid ? grid : 256;
   g = std::min<uint64_t>(g, std::max<uint64_t>(blocks, a.heavy_count));
   g = std::max<uint64_t>(8, (g + 7) / 8 * 8);
-  window_kernel<R, TEST><<<uint32_t(g), kWinNT, 0, s>>>(a);
+  window_kernel<R, TEST, SYS><<<uint32_t(g), kWinNT, 0, s>>>(a);
   return hipGetLastError();
 }
 
 hipError_t launch_window_union(int r, int log2m, const UnionArgs &a, hipStream_t s) {
   if (!window_union_supported(r, log2m) || !a.indeg) return hipErrorInvalidValue;
+  const bool sys = a.flags & (F_SYSTOLIC | F_LIST);
   const bool test = a.flags & F_UNTESTED ? false : true;
-  if (r == 5) return test ? launch_window_t<5, true>(a, s) : launch_window_t<5, false>(a, s);
-  return test ? launch_window_t<6, true>(a, s) : launch_window_t<6, false>(a, s);
+  if (r == 5)
+    return sys ? launch_window_t<5, true, true>(a, s)
+               : test ? launch_window_t<5, true, false>(a, s) : launch_window_t<5, false, false>(a, s);
+  return sys ? launch_window_t<6, true, true>(a, s)
+             : test ? launch_window_t<6, true, false>(a, s) : launch_window_t<6, false, false>(a, s);
 }
 
 hipError_t launch_indegree(const uint64_t *off, const uint32_t *succ, uint64_t n, uint32_t *indeg,

@@ -625,10 +625,11 @@ class HyperBall:
 
     def set_window(self, mode: int, block_nodes: int = 0, half_width: int = 0,
                    heavy_degree: int = 0) -> None:
-        """Window kernel for log2m-6 counters: 0 or 1 in the iterations that
-        check every node, 2 never; block_nodes / half_width cap its block and
-        tile (0: the kernel's own); rows longer than heavy_degree take a
-        workgroup (0: the default, 2**32 - 1: none) (hb_set_window)."""
+        """Window kernel for log2m-6 counters: 0 by the library's rule (the
+        iterations that check every node and large worklists), 1 whenever it
+        can run, 2 never; block_nodes / half_width cap its block and tile (0:
+        the kernel's own); rows longer than heavy_degree take a workgroup (0:
+        the default, 2**32 - 1: none) (hb_set_window)."""
         _check(lib().hb_set_window(self._h, int(mode), int(block_nodes), int(half_width),
                                    int(heavy_degree)))
 
