@@ -193,6 +193,16 @@ def bench_batch(hbm, args):
         return sum(s["visited_arcs"] * h.params()["num_registers"]
                    for h in instances for s in h.iteration_stats())
 
+    def bytes_of(instances):
+        # the headline's no-reuse model per instance: C = 4 r m / 32 bytes,
+        # slabs of 4 r bytes
+        total = 0.0
+        for h, b in zip(instances, builders):
+            C = 4 * args.reg_bits * h.params()["num_registers"] // 32
+            total += sum(algorithmic_bytes(s, b._graph.num_nodes, C, 4 * args.reg_bits, 2)
+                         for s in h.iteration_stats())
+        return total
+
     batch = hbm.HyperBallBatch(builders)
     if args.heavy_threshold is None:
         chosen, table = batch.autotune()
@@ -210,6 +220,7 @@ def bench_batch(hbm, args):
         batch.run_until_done()
     batched_s = (time.perf_counter() - t0) / args.steps
     work = work_of(batch.instances)
+    nbytes = bytes_of(batch.instances)
     steps_ms = batch.step_times()
     nf_batch = [h.neighborhood_function() for h in batch.instances]
     batch.close()
@@ -242,7 +253,15 @@ def bench_batch(hbm, args):
             "sequential_value": work / seq_s, "sequential_ms": seq_s * 1e3,
             "speedup": seq_s / batched_s, "identical_results": bool(same),
             "batch_heavy_row_threshold": chosen,
-            "unit": "arc-registers/s"}
+            "unit": "arc-registers/s",
+            "roofline": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                         "algorithmic_bytes_per_run": nbytes,
+                         "batched_achieved": nbytes / batched_s / 1e9,
+                         "batched_frac": nbytes / batched_s / 1e9 / PEAK_HBM_GBS,
+                         "sequential_achieved": nbytes / seq_s / 1e9,
+                         "sequential_frac": nbytes / seq_s / 1e9 / PEAK_HBM_GBS,
+                         "note": "every instance's iterations under the headline's model, "
+                                 "over the wall time of a whole run (host work included)"}}
 
 
 def host_threads():

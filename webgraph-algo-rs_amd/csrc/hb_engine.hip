@@ -122,6 +122,7 @@ struct Instance {
   DevBuf<uint32_t> succ, tsucc;
   DevBuf<uint32_t> cnt[2], mod[2], must[2];
   DevBuf<uint32_t> chg[2];  // change masks, swapped with the modified bitmaps
+  DevBuf<uint32_t> seed_items;  // unweighted: each node's seeded register (F_SEEDED reads it)
   uint32_t chg_shift = 0;   // log2 slabs per change-mask bit
   // Delta gathering (read only the changed slabs of successors) pays when
   // counters are large and few of their slabs changed: it costs a 4-byte mask
@@ -290,6 +291,7 @@ struct Instance {
     s.weight_seed = weight_seed;
     s.chg = chg[0].p;
     s.chg_shift = chg_shift;
+    s.items = seed_items.p;
     if (weighted) {
       HB_HIP(launch_seed(s, stream));
       HB_HIP(launch_count_all(int(r), int(log2m), cnt[0].p, n, amm, m, lc.p, cache.p, stream));
@@ -401,11 +403,8 @@ struct Instance {
     ua.dist = double(iteration + 1);
     ua.alpha_m_m = amm;
     ua.m = m;
-    ua.log2m = log2m;
-    ua.hash = hash;
-    ua.seed = seed;
-    ua.sentinel = sentinel;
     ua.lc_table = lc.p;
+    ua.seed_items = seed_items.p;
     ua.stats = stats.p;
     ua.stat_slots = stat_slots.p;
     ua.work = work.p;
@@ -599,7 +598,7 @@ struct Instance {
   bool stream_this_iteration() const {
     if (counter_memory != HB_COUNTERS_STREAM || stream_mode == 2) return false;
     if (stream_mode == 1) return true;
-    if (iteration == 0) return weighted;  // seeded registers come from hashes: nothing to read
+    if (iteration == 0) return weighted;  // the seeded iteration reads no counters
     const double visited = double(modified) * double(a) / double(std::max<uint64_t>(n, 1));
     const double streamed = double(n) * double(tiles > 3 ? tiles - 2 : 1);
     return visited >= streamed;
@@ -1126,6 +1125,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
       I.must[k].alloc(I.bitmap_words);
       I.chg[k].alloc(n);
     }
+    if (!I.weighted) I.seed_items.alloc(n);
     I.cache.alloc(n);
     if (I.do_sd) I.sd.alloc(n * I.acc_bytes());
     if (I.do_hc) I.hc.alloc(n * I.acc_bytes());

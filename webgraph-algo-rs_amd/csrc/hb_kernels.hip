@@ -429,6 +429,32 @@ __device__ __forceinline__ void seed_register(uint32_t hash, uint64_t seed, uint
   val = uint32_t(__builtin_ctzll((x >> log2m) | sentinel)) + 1;
 }
 
+// Raises register (w & 0xffff) of a lane's slabs (slab0 .. slab0 + SPL - 1,
+// R planes each, at cur) to the value w >> 16 if that is larger: one packed
+// seeded item.
+template <int R, int SPL>
+__device__ __forceinline__ void seed_apply(uint32_t *cur, uint32_t w, uint32_t slab0) {
+  const uint32_t reg = w & 0xffffu, val = w >> 16, q = (reg >> 5) - slab0, bit = reg & 31;
+  uint32_t plane[R];
+#pragma unroll
+  for (int b = 0; b < R; b++) {
+    plane[b] = cur[b];
+#pragma unroll
+    for (int qq = 1; qq < SPL; qq++) plane[b] = q == uint32_t(qq) ? cur[qq * R + b] : plane[b];
+  }
+  uint32_t cv = 0;
+#pragma unroll
+  for (int b = 0; b < R; b++) cv |= ((plane[b] >> bit) & 1u) << b;
+  if (val > cv) {
+#pragma unroll
+    for (int b = 0; b < R; b++) {
+      const uint32_t np = (plane[b] & ~(1u << bit)) | (((val >> b) & 1u) << bit);
+#pragma unroll
+      for (int qq = 0; qq < SPL; qq++) cur[qq * R + b] = q == uint32_t(qq) ? np : cur[qq * R + b];
+    }
+  }
+}
+
 // Merges into `cur` the counters of the modified successors (other than v)
 // found in the slices of v's row that start at first, first + stride, ...
 // (hyperball_impl.rs:1037-1049).  `prev` / `cur` are loaded from the node's
@@ -448,10 +474,10 @@ __device__ __forceinline__ void seed_register(uint32_t hash, uint64_t seed, uint
 // In iteration 0 the mask holds the slabs with a non-zero register.
 //
 // Seeded counters (MODE 2, iteration 0 without weights): every successor's
-// counter holds the single register of its own hash, so each lane hashes the
-// ids it read, packs (register, value) into the slot instead of the id, and
-// the lane owning that register's slab takes the maximum.  No successor
-// counter or change mask is read.
+// counter holds the single register of its own hash, which seeding left in
+// seed_items as (register, value); each lane reads those of the ids it read
+// into the slot instead of the id, and the lane owning that register's slab
+// takes the maximum.  No successor counter or change mask is read.
 //
 // Untested (MODE 3, window iterations): every successor other than v is
 // merged, modified or not.  This is exact when the iteration checks every
@@ -506,11 +532,7 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
       }
       if (SEEDED) {
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-          uint32_t reg = 0, val = 0;
-          if (f[j]) seed_register(a.hash, a.seed, a.log2m, a.sentinel, s[j], reg, val);
-          mk[j] = reg | (val << 16);
-        }
+        for (int j = 0; j < K; j++) mk[j] = f[j] ? a.seed_items[s[j]] : 0u;
       }
     } else if (speculate) {
 #pragma unroll
@@ -539,9 +561,8 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
     if (!loaded) {
       if constexpr (SEEDED) {
         // the node's own counter holds the one register of its own hash
-        uint32_t oreg = 0, oval = 0;
-        seed_register(a.hash, a.seed, a.log2m, a.sentinel, v, oreg, oval);
-        const uint32_t oslab = oreg >> 5, obit = oreg & 31;
+        const uint32_t ow = a.seed_items[v];
+        const uint32_t oval = ow >> 16, oslab = (ow & 0xffffu) >> 5, obit = ow & 31;
 #pragma unroll
         for (int q = 0; q < SPL; q++)
 #pragma unroll
@@ -563,22 +584,24 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
         uint32_t w8[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) w8[u] = ids[k0 + u];
+        if constexpr (G >= 8) {
+          // each lane marks the slots whose register lies in its slabs (one in
+          // G), then applies only those: a wavefront runs the register update
+          // as often as its busiest lane has hits, not once per slot
+          uint32_t hits = 0;
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          if (k0 + u >= cnt) break;
-          const uint32_t w = w8[u];
-          const uint32_t reg = w & 0xffffu, val = w >> 16, slab = reg >> 5, bit = reg & 31;
+          for (int u = 0; u < 8; u++)
+            if (k0 + u < cnt && ((w8[u] & 0xffffu) >> 5) - slab0 < uint32_t(SPL)) hits |= 1u << u;
+          while (hits) {
+            const uint32_t w = ids[k0 + __builtin_ctz(hits)];
+            hits &= hits - 1;
+            seed_apply<R, SPL>(cur, w, slab0);
+          }
+        } else {
 #pragma unroll
-          for (int q = 0; q < SPL; q++) {
-            if (slab != slab0 + q) continue;
-            uint32_t cv = 0;
-#pragma unroll
-            for (int b = 0; b < R; b++) cv |= ((cur[q * R + b] >> bit) & 1u) << b;
-            if (val > cv) {
-#pragma unroll
-              for (int b = 0; b < R; b++)
-                cur[q * R + b] = (cur[q * R + b] & ~(1u << bit)) | (((val >> b) & 1u) << bit);
-            }
+          for (int u = 0; u < 8; u++) {
+            if (k0 + u >= cnt) break;
+            if (((w8[u] & 0xffffu) >> 5) - slab0 < uint32_t(SPL)) seed_apply<R, SPL>(cur, w8[u], slab0);
           }
         }
       }
@@ -1514,6 +1537,7 @@ __global__ __launch_bounds__(kBlock) void seed_count_kernel(SeedArgs a, uint32_t
       if (gl == 0) {
         out[i] = lc[m - 1];
         a.chg[i] = 1u << ((reg >> 5) >> a.chg_shift);
+        if (a.items) a.items[i] = reg | (val << 16);
       }
     }
   }

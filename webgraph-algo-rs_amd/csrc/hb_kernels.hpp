@@ -108,9 +108,8 @@ struct UnionArgs {
   double dist;  // iteration + 1
   double alpha_m_m;
   uint32_t m;
-  uint32_t log2m, hash;    // F_SEEDED: the seeding hash (HB_HASH_*), its seed and the
-  uint64_t seed, sentinel; // register-value sentinel, as in SeedArgs
   const double *lc_table;  // m * ln(m / z) for z = 0..m
+  const uint32_t *seed_items;  // F_SEEDED: each node's seeded register, index | value << 16
   Stats *stats;            // zeroed before the launch
   uint64_t *stat_slots;    // kStatSlots * kStatSlotWords, zeroed before the launch; null:
                            // wavefronts add to *stats directly
@@ -159,6 +158,7 @@ struct SeedArgs {
   uint64_t weight_seed;
   uint32_t *chg;        // per node: the slabs holding a non-zero register
   uint32_t chg_shift;
+  uint32_t *items;      // nullable (seed_count only): each node's register, index | value << 16
 };
 
 // ---- batched execution --------------------------------------------------
