@@ -515,24 +515,31 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
   // with every node checked most successors are modified: read the masks
   // together with the modified bits instead of after them
   const bool speculate = a.flags & F_CHECK_ALL;
-  for (uint64_t base = first; base < end; base += stride) {
-    uint32_t s[K], mk[K];
-    bool f[K];
+  // Seeded counters need no counter reads and leave the mask half of the slot
+  // unused: a pass takes two of the group's slices (KS ids per lane, packed
+  // pairs filling the whole slot), and the node's own pair is read up front,
+  // so that a short row costs two dependent reads (ids, then pairs).
+  constexpr int KS = SEEDED ? 2 * K : K;
+  const uint64_t step = SEEDED ? 2 * stride : stride;
+  const uint32_t own_item = SEEDED ? a.seed_items[v] : 0u;
+  for (uint64_t base = first; base < end; base += step) {
+    uint32_t s[KS], mk[KS];
+    bool f[KS];
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-      const uint64_t idx = base + uint64_t(j) * G + gl;
+    for (int j = 0; j < KS; j++) {
+      const uint64_t idx = base + uint64_t(j / K) * stride + uint64_t(j % K) * G + gl;
       s[j] = idx < end ? a.succ[idx] : v;
     }
     if (!DELTA) {
 #pragma unroll
-      for (int j = 0; j < K; j++) {
+      for (int j = 0; j < KS; j++) {
         // seeded counters: init marks every node modified
         f[j] = (s[j] != v) && (SEEDED || UNTESTED || test_bit(a.curr_mod, s[j]));
         mk[j] = ~0u;
       }
       if (SEEDED) {
 #pragma unroll
-        for (int j = 0; j < K; j++) mk[j] = f[j] ? a.seed_items[s[j]] : 0u;
+        for (int j = 0; j < KS; j++) mk[j] = f[j] ? a.seed_items[s[j]] : 0u;
       }
     } else if (speculate) {
 #pragma unroll
@@ -547,7 +554,7 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
     }
     uint32_t cnt = 0;
 #pragma unroll
-    for (int j = 0; j < K; j++) {
+    for (int j = 0; j < KS; j++) {
       const uint64_t mask = group_ballot<G>(f[j]);
       if (f[j]) {
         const uint32_t pos = cnt + __popcll(mask & below);
@@ -561,8 +568,8 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
     if (!loaded) {
       if constexpr (SEEDED) {
         // the node's own counter holds the one register of its own hash
-        const uint32_t ow = a.seed_items[v];
-        const uint32_t oval = ow >> 16, oslab = (ow & 0xffffu) >> 5, obit = ow & 31;
+        const uint32_t oval = own_item >> 16, oslab = (own_item & 0xffffu) >> 5,
+                       obit = own_item & 31;
 #pragma unroll
         for (int q = 0; q < SPL; q++)
 #pragma unroll
@@ -577,9 +584,10 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
     }
     nvisit += cnt;
     if (SEEDED) {
-      // eight slots per batch, read before use; a batch may read up to seven
-      // slots past cnt, inside the group's 2 * slice_width<G>() words
-      static_assert(slice_width<G>() >= 8, "seeded batches read within the slot");
+      // eight slots per batch, read before use; cnt is at most the slot's
+      // 2 * slice_width<G>() words, a multiple of 8, so a batch that starts
+      // below cnt reads inside the slot
+      static_assert(slice_width<G>() % 4 == 0, "seeded batches read within the slot");
       for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
         uint32_t w8[8];
 #pragma unroll
