@@ -22,7 +22,7 @@ CSRC = os.path.join(REPO, "webgraph-algo-rs_amd", "csrc")
 OUT = os.path.join(HERE, "build")
 FLAGS = ["-std=c++20", "-g", "-O1", "-fno-omit-frame-pointer", "-pthread",
          "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
-         "-ffp-contract=off", "-DHB_EMULATION=1", "-w"]
+         "-ffp-contract=off", "-DHB_EMULATION=1", "-DHB_SEED_ITEMS_MAX_NODES=1000", "-w"]
 
 
 def _matching_paren(text, i):

@@ -26,6 +26,13 @@
 #include "hb_hll.hpp"
 #include "hb_kernels.hpp"
 
+// Nodes up to which iteration 0 reads seeded pairs instead of hashing (4 bytes
+// a node: 128 MiB, half the last-level cache).  The CPU emulation lowers it so
+// that its small graphs take both paths.
+#ifndef HB_SEED_ITEMS_MAX_NODES
+#define HB_SEED_ITEMS_MAX_NODES (uint64_t(1) << 25)
+#endif
+
 namespace hb {
 namespace {
 
@@ -122,7 +129,9 @@ struct Instance {
   DevBuf<uint32_t> succ, tsucc;
   DevBuf<uint32_t> cnt[2], mod[2], must[2];
   DevBuf<uint32_t> chg[2];  // change masks, swapped with the modified bitmaps
-  DevBuf<uint32_t> seed_items;  // unweighted: each node's seeded register (F_SEEDED reads it)
+  // unweighted, n <= HB_SEED_ITEMS_MAX_NODES: each node's seeded register
+  // (F_SEEDED reads it)
+  DevBuf<uint32_t> seed_items;
   uint32_t chg_shift = 0;   // log2 slabs per change-mask bit
   // Delta gathering (read only the changed slabs of successors) pays when
   // counters are large and few of their slabs changed: it costs a 4-byte mask
@@ -405,6 +414,10 @@ struct Instance {
     ua.m = m;
     ua.lc_table = lc.p;
     ua.seed_items = seed_items.p;
+    ua.log2m = log2m;
+    ua.hash = hash;
+    ua.seed = seed;
+    ua.sentinel = sentinel;
     ua.stats = stats.p;
     ua.stat_slots = stat_slots.p;
     ua.work = work.p;
@@ -1125,7 +1138,9 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
       I.must[k].alloc(I.bitmap_words);
       I.chg[k].alloc(n);
     }
-    if (!I.weighted) I.seed_items.alloc(n);
+    // seeded pairs for iteration 0 only while they stay in the last-level
+    // cache (random reads of a larger array cost more than hashing again)
+    if (!I.weighted && n <= HB_SEED_ITEMS_MAX_NODES) I.seed_items.alloc(n);
     I.cache.alloc(n);
     if (I.do_sd) I.sd.alloc(n * I.acc_bytes());
     if (I.do_hc) I.hc.alloc(n * I.acc_bytes());

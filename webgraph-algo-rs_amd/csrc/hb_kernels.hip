@@ -429,6 +429,13 @@ __device__ __forceinline__ void seed_register(uint32_t hash, uint64_t seed, uint
   val = uint32_t(__builtin_ctzll((x >> log2m) | sentinel)) + 1;
 }
 
+// The seeded register of node `item`, packed as index | value << 16.
+__device__ __forceinline__ uint32_t seed_item(const UnionArgs &a, uint32_t item) {
+  uint32_t reg = 0, val = 0;
+  seed_register(a.hash, a.seed, a.log2m, a.sentinel, item, reg, val);
+  return reg | (val << 16);
+}
+
 // Raises register (w & 0xffff) of a lane's slabs (slab0 .. slab0 + SPL - 1,
 // R planes each, at cur) to the value w >> 16 if that is larger: one packed
 // seeded item.
@@ -475,9 +482,11 @@ __device__ __forceinline__ void seed_apply(uint32_t *cur, uint32_t w, uint32_t s
 //
 // Seeded counters (MODE 2, iteration 0 without weights): every successor's
 // counter holds the single register of its own hash, which seeding left in
-// seed_items as (register, value); each lane reads those of the ids it read
-// into the slot instead of the id, and the lane owning that register's slab
-// takes the maximum.  No successor counter or change mask is read.
+// seed_items as (register, value) when the array is small enough to stay in
+// the caches (else the lanes hash the ids again); each lane puts those of the
+// ids it read into the slot instead of the id, and the lane owning that
+// register's slab takes the maximum.  No successor counter or change mask is
+// read.
 //
 // Untested (MODE 3, window iterations): every successor other than v is
 // merged, modified or not.  This is exact when the iteration checks every
@@ -521,7 +530,7 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
   // so that a short row costs two dependent reads (ids, then pairs).
   constexpr int KS = SEEDED ? 2 * K : K;
   const uint64_t step = SEEDED ? 2 * stride : stride;
-  const uint32_t own_item = SEEDED ? a.seed_items[v] : 0u;
+  const uint32_t own_item = !SEEDED ? 0u : a.seed_items ? a.seed_items[v] : seed_item(a, v);
   for (uint64_t base = first; base < end; base += step) {
     uint32_t s[KS], mk[KS];
     bool f[KS];
@@ -538,8 +547,13 @@ __device__ __forceinline__ void merge_slices(const UnionArgs &a, uint32_t v, int
         mk[j] = ~0u;
       }
       if (SEEDED) {
+        if (a.seed_items) {
 #pragma unroll
-        for (int j = 0; j < KS; j++) mk[j] = f[j] ? a.seed_items[s[j]] : 0u;
+          for (int j = 0; j < KS; j++) mk[j] = f[j] ? a.seed_items[s[j]] : 0u;
+        } else {
+#pragma unroll
+          for (int j = 0; j < KS; j++) mk[j] = f[j] ? seed_item(a, s[j]) : 0u;
+        }
       }
     } else if (speculate) {
 #pragma unroll

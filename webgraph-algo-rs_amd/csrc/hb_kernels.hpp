@@ -109,7 +109,11 @@ struct UnionArgs {
   double alpha_m_m;
   uint32_t m;
   const double *lc_table;  // m * ln(m / z) for z = 0..m
-  const uint32_t *seed_items;  // F_SEEDED: each node's seeded register, index | value << 16
+  // F_SEEDED: each node's seeded register, index | value << 16, or null to
+  // hash the node ids again (hash, seed, log2m and sentinel as in SeedArgs)
+  const uint32_t *seed_items;
+  uint32_t log2m, hash;
+  uint64_t seed, sentinel;
   Stats *stats;            // zeroed before the launch
   uint64_t *stat_slots;    // kStatSlots * kStatSlotWords, zeroed before the launch; null:
                            // wavefronts add to *stats directly
