@@ -347,7 +347,8 @@ hb_status hb_set_window(hb_instance *hb, int mode, uint32_t block_nodes, uint32_
 hb_status hb_set_streaming(hb_instance *hb, int mode);
 /* Autotuner of that split (in the role of the reference's arc_granularity
  * tuning knob, hyperball_impl.rs:271-274, which sizes CPU work units): times
- * the first, dense iteration under each threshold 64 * 2^k that changes the
+ * the first iteration that gathers whole counters (iteration 1, every node
+ * checked, delta gathering off) under each threshold 64 * 2^k that changes the
  * set of heavy rows, and "none" (UINT32_MAX), best of `reps`; keeps the
  * fastest.  Writes up to `cap` (threshold, ms) pairs, their count to *len and
  * the chosen threshold to *chosen (each pointer may be NULL).  Resets the run
@@ -374,7 +375,7 @@ hb_status hb_batch_instance(hb_batch *b, size_t i, hb_instance **out);
 hb_status hb_batch_run(hb_batch *b, uint64_t upper_bound, double threshold);
 hb_status hb_batch_run_until_done(hb_batch *b);
 /* hb_autotune for a batch: one threshold for every instance, timed on the
- * first batched iteration (same outputs and reset as hb_autotune). */
+ * second batched iteration (same outputs and reset as hb_autotune). */
 hb_status hb_batch_autotune(hb_batch *b, uint32_t reps, uint32_t *thresholds, double *ms,
                             size_t cap, size_t *len, uint32_t *chosen);
 /* HIP-graph replay of the batched iteration on (default) or off. */

@@ -233,11 +233,13 @@ struct Instance {
     heavy_count = k;
   }
 
-  // Light/heavy split autotuner: times the first (dense, every node checked)
-  // iteration under each candidate threshold that changes the heavy-row set
-  // and keeps the fastest.  Results do not depend on the threshold; the run
-  // state is reset as by init().  Candidates: every 64 * 2^k that leaves a
-  // different, non-empty set of heavy rows, then "no heavy rows".
+  // Light/heavy split autotuner: times the first iteration that gathers
+  // whole successor counters (iteration 1, every node checked, delta
+  // gathering off; iteration 0 reads seeded registers only, unless weighted)
+  // under each candidate threshold that changes the heavy-row set and keeps
+  // the fastest.  Results do not depend on the threshold; the run state is
+  // reset as by init().  Candidates: every 64 * 2^k that leaves a different,
+  // non-empty set of heavy rows, then "no heavy rows".
   std::vector<std::pair<uint64_t, double>> autotune(uint32_t reps) {
     if (reps == 0) reps = 1;
     std::vector<uint64_t> cand;
@@ -250,7 +252,9 @@ struct Instance {
     }
     cand.push_back(UINT64_MAX);
     const bool was_timing = timing;
+    const int was_delta = delta_mode;
     timing = true;
+    delta_mode = 2;
     std::vector<std::pair<uint64_t, double>> table;
     try {
       for (uint64_t t : cand) {
@@ -259,15 +263,18 @@ struct Instance {
         for (uint32_t r = 0; r < reps; r++) {
           init();
           iterate();
+          iterate();
           best = std::min(best, history.back().union_ms);
         }
         table.push_back({t, best});
       }
     } catch (...) {
       timing = was_timing;
+      delta_mode = was_delta;
       throw;
     }
     timing = was_timing;
+    delta_mode = was_delta;
     auto it = std::min_element(table.begin(), table.end(),
                                [](const auto &a, const auto &b) { return a.second < b.second; });
     set_heavy_degree(it->first);
@@ -963,8 +970,10 @@ struct Batch {
   }
 
   // Light/heavy split autotuner for the batch: one threshold for every
-  // instance, chosen by timing the first batched iteration (all instances
-  // dense) under each 64 * 2^k below the longest row, and "no heavy rows".
+  // instance, chosen by timing the second batched iteration (all instances
+  // dense and gathering whole counters: delta gathering off; the first reads
+  // seeded registers only) under each 64 * 2^k below the longest row, and
+  // "no heavy rows".
   std::vector<std::pair<uint64_t, double>> autotune(uint32_t reps) {
     if (reps == 0) reps = 1;
     uint64_t top = 0;
@@ -974,16 +983,31 @@ struct Batch {
     for (uint64_t t = Instance::kHeavyFloor; t < top; t *= 2) cand.push_back(t);
     cand.push_back(UINT64_MAX);
     std::vector<std::pair<uint64_t, double>> table;
-    for (uint64_t t : cand) {
-      for (auto &I : inst) I->set_heavy_degree(t);
-      double best = 1e300;
-      for (uint32_t r = 0; r < reps; r++) {
-        begin(UINT64_MAX, NAN);
-        advance();
-        best = std::min(best, step_ms.back());
-      }
-      table.push_back({t, best});
+    std::vector<int> was_delta;
+    for (auto &I : inst) {
+      was_delta.push_back(I->delta_mode);
+      I->delta_mode = 2;
     }
+    auto restore = [&] {
+      for (size_t i = 0; i < inst.size(); i++) inst[i]->delta_mode = was_delta[i];
+    };
+    try {
+      for (uint64_t t : cand) {
+        for (auto &I : inst) I->set_heavy_degree(t);
+        double best = 1e300;
+        for (uint32_t r = 0; r < reps; r++) {
+          begin(UINT64_MAX, NAN);
+          advance();
+          advance();  // does nothing when every instance stopped after the first
+          if (!step_ms.empty()) best = std::min(best, step_ms.back());
+        }
+        table.push_back({t, best});
+      }
+    } catch (...) {
+      restore();
+      throw;
+    }
+    restore();
     auto it = std::min_element(table.begin(), table.end(),
                                [](const auto &a, const auto &b) { return a.second < b.second; });
     for (auto &I : inst) {

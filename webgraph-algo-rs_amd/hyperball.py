@@ -589,8 +589,9 @@ class HyperBall:
         _check(lib().hb_set_timing(self._h, int(on)))
 
     def autotune(self, reps: int = 2) -> tuple[int, list[tuple[int, float]]]:
-        """Chooses the light/heavy row split by timing the dense first
-        iteration; returns (chosen threshold, [(threshold, ms), ...]).
+        """Chooses the light/heavy row split by timing the first iteration
+        that gathers whole counters (iteration 1, delta gathering off);
+        returns (chosen threshold, [(threshold, ms), ...]).
         2**32 - 1 stands for "no heavy rows".  Resets the run state."""
         cap = 64
         th = np.zeros(cap, dtype=np.uint32)
@@ -682,8 +683,9 @@ class HyperBallBatch:
         _check(lib().hb_batch_set_graph(self._h, int(on)))
 
     def autotune(self, reps: int = 2) -> tuple[int, list[tuple[int, float]]]:
-        """One light/heavy split for all instances, timed on the first
-        batched iteration; returns (chosen, [(threshold, ms), ...])."""
+        """One light/heavy split for all instances, timed on the second
+        batched iteration (delta gathering off); returns (chosen,
+        [(threshold, ms), ...])."""
         cap = 64
         th = np.zeros(cap, dtype=np.uint32)
         ms = np.zeros(cap)
