@@ -39,7 +39,7 @@ enum UnionFlags : uint32_t {
   F_LIST = 8u,        // nodes come from a compacted list whose length lives on the device
   F_DELTA = 16u,      // gather only the changed slabs of successors (change masks)
   F_SEEDED = 32u,     // counters are as seeded without weights: each holds the one register
-                      // of its node's hash, computed instead of read
+                      // of its node's hash, taken from seed_items (or hashed) instead of read
   F_UNTESTED = 64u,   // window kernel: successors outside the tile (and those of rows that
                       // take a workgroup) are merged without testing their modified bits
 };
