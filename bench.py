@@ -294,10 +294,18 @@ def small_uniform(hbm, orc, args):
     gpu_s = (time.perf_counter() - t0) / reps
     st = hb.iteration_stats()
     work = sum(s["visited_arcs"] for s in st) * m
+    r = hb.params()["register_bits"]
+    nbytes = sum(algorithmic_bytes(s, g.num_nodes, 4 * r * m // 32, 4 * r, 2) for s in st)
     nf, sd = hb.neighborhood_function(), hb.sum_of_distances()
     hb.close()
     out = {"workload": "uniform graph n=100000, 8 successors each, log2m 8, r from n, SipHash-1-3",
-           "iterations": len(st), "gpu": {"value": work / gpu_s, "ms_per_run": gpu_s * 1e3}}
+           "iterations": len(st), "gpu": {"value": work / gpu_s, "ms_per_run": gpu_s * 1e3,
+                                          "ms_per_iteration": gpu_s * 1e3 / max(1, len(st))},
+           "roofline": {"bound": "latency", "achieved": nbytes / gpu_s / 1e9, "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": nbytes / gpu_s / 1e9 / PEAK_HBM_GBS,
+                        "algorithmic_bytes_per_run": nbytes,
+                        "note": "the headline's byte model over the whole run's wall time; each "
+                                "iteration is a few launches and one host synchronisation"}}
     if orc is not None:
         threads = host_threads()
         ref = orc.OracleHyperBall(g.offsets, g.successors, gt.offsets, gt.successors, log2m=8,
