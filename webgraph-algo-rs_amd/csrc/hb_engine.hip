@@ -156,7 +156,7 @@ struct Instance {
   // inside their block.  Successors outside the tile are read untested while
   // at least kUntestedShare of the nodes changed in the last iteration.
   static constexpr uint64_t kWindowHeavy = 8192;
-  static constexpr double kUntestedShare = 0.9;
+  static constexpr double kUntestedShare = 0.5;
   static constexpr double kWindowListShare = 0.125;
   DevBuf<uint32_t> indeg;
   int window_mode = 0;  // 0: by the rule above, 1: whenever the kernel can run, 2: never
