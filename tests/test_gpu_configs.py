@@ -243,7 +243,8 @@ def test_large_shape_small_n(hbm, orc):
 def test_large_instance_full_size_properties(hbm, orc):
     """The large instance at full size: n = 10^8, about 2.1e9 arcs, transpose
     built on the device, log2m 6, 5-bit registers, FP32 Kahan accumulators.
-    At the fixed point every counter dominates its successors' (sampled),
+    Iteration 0 equals the oracle's seeding on sampled nodes.  At the fixed
+    point every counter dominates its successors' (sampled),
     one more iteration modifies nothing, the neighbourhood function starts
     at n and never decreases, reachable counts equal a fresh count() of the
     counters, and a second run reproduces every value bit for bit."""
@@ -253,6 +254,20 @@ def test_large_instance_full_size_properties(hbm, orc):
     hb = (hbm.HyperBallBuilder.with_hyper_log_log(g, None, 6).transpose_on_device()
           .register_bits(5).jenkins(42).sum_of_distances(True).sum_of_inverse_distances(True)
           .accumulator_precision(hbm.ACC_F32_KAHAN).build())
+    hll = orc.Hll(6, 5, n, orc.HASH_JENKINS, 42)
+    rng = np.random.default_rng(1)
+    # Iteration 0 above 2^25 nodes hashes the successor ids again (no array of
+    # seeded registers): sampled counters equal the oracle's seeding of the
+    # node and its successors.
+    hb.init()
+    hb.iterate()
+    cnt = hb.counters(0)
+    for v in rng.choice(n, 300, replace=False):
+        want = hll.new_counter()
+        for item in [v, *g.successors[g.offsets[v]:g.offsets[v + 1]]]:
+            hll.add(want, int(item))
+        assert np.array_equal(cnt[v], want), f"iteration 0: counter {v}"
+    del cnt
     hb.run_until_done()
     nf = hb.neighborhood_function()
     sd, hc = hb.sum_of_distances(), hb.harmonic_centralities()
@@ -261,8 +276,6 @@ def test_large_instance_full_size_properties(hbm, orc):
     hb.iterate()
     assert hb.iteration_stats()[-1]["modified"] == 0
     cnt = hb.counters(0)
-    hll = orc.Hll(6, 5, n, orc.HASH_JENKINS, 42)
-    rng = np.random.default_rng(1)
     for v in rng.choice(n, 300, replace=False):
         x = cnt[v].copy()
         for s in g.successors[g.offsets[v]:g.offsets[v + 1]][:50]:
