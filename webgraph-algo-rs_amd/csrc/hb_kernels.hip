@@ -932,9 +932,11 @@ __device__ __forceinline__ void run_chunk(const UnionArgs &a, uint64_t chunk, ui
 // NT * D words, s_ids 2 * NT * ids_per_lane<G>() words (a slot per lane
 // group, as in the chunk phase), s_sum two words (any lane group loaded the
 // counter; visited arcs).  Without COUNT the node's visited arcs and gathered
-// slabs are not counted (the window kernel counts them from in-degrees), and
-// under F_UNTESTED its successors are merged untested (merge_slices MODE 3).
-template <int R, int G, int SPL, int NT = kBlock, bool COUNT = true>
+// slabs are not counted (the window kernel counts them from in-degrees).  WIN
+// (window iterations: neither seeded counters nor delta gathering) compiles
+// only the whole-counter merges, untested under F_UNTESTED (merge_slices
+// MODE 3).
+template <int R, int G, int SPL, int NT = kBlock, bool COUNT = true, bool WIN = false>
 __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint32_t *s_acc,
                                            uint32_t *s_ids, uint32_t *s_sum,
                                            Acc &acc) {
@@ -952,8 +954,11 @@ __device__ __forceinline__ void heavy_node(const UnionArgs &a, uint32_t v, uint3
     const uint64_t first = beg + uint64_t(g) * slice_width<G>();
     const uint64_t stride = uint64_t(NG) * slice_width<G>();
     uint32_t *slot = s_ids + g * 2 * slice_width<G>();
-    if (!COUNT && (a.flags & F_UNTESTED))
+    if (WIN && (a.flags & F_UNTESTED))
       merge_slices<R, G, SPL, 3>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
+                                 gathered, slot);
+    else if (WIN)
+      merge_slices<R, G, SPL, 0>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
                                  gathered, slot);
     else
       merge_successors<R, G, SPL>(a, v, gl, first, end, stride, prev, cur, loaded, nvisit,
@@ -1317,11 +1322,11 @@ template <int R>
 __device__ __forceinline__ void win_heavy_row(const UnionArgs &a, uint64_t h, WinLds<R> &L,
                                               Acc &acc) {
   if (a.flags & (F_SYSTOLIC | F_LIST))  // worklist iterations count visited arcs per row
-    heavy_node<R, 2, 1, kWinNT, true>(a, a.heavy_list[h], L.u.heavy.acc, L.u.heavy.ids,
-                                      L.u.heavy.sum, acc);
+    heavy_node<R, 2, 1, kWinNT, true, true>(a, a.heavy_list[h], L.u.heavy.acc,
+                                            L.u.heavy.ids, L.u.heavy.sum, acc);
   else
-    heavy_node<R, 2, 1, kWinNT, false>(a, a.heavy_list[h], L.u.heavy.acc, L.u.heavy.ids,
-                                       L.u.heavy.sum, acc);
+    heavy_node<R, 2, 1, kWinNT, false, true>(a, a.heavy_list[h], L.u.heavy.acc,
+                                             L.u.heavy.ids, L.u.heavy.sum, acc);
 }
 
 // Block blk of the instance a: stage its tile, rank its nodes, work on them.
@@ -2358,7 +2363,14 @@ bool union_shape(int r, int log2m, int *lanes, int *spl, int *nseg) {
   return false;
 }
 
+// The seeded iteration 0 reads no counters, so at log2m 10 it takes 8 lanes
+// per node (4 slabs each) instead of 16: half the lanes wait on each short
+// row (headline iteration 0: 7.17 -> 6.54 ms at r = 6, 7.02 -> 6.07 at r = 5).
 hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s) {
+  if ((a.flags & F_SEEDED) && log2m == 10) {
+    if (r == 5) return launch_union_t<5, 8, 4, 1>(a, grid, s);
+    if (r == 6) return launch_union_t<6, 8, 4, 1>(a, grid, s);
+  }
 #define X(RR, LM, GG, SS, NS) \
   if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS, NS>(a, grid, s);
   HB_SHAPES(X)
