@@ -2363,14 +2363,19 @@ bool union_shape(int r, int log2m, int *lanes, int *spl, int *nseg) {
   return false;
 }
 
-// The seeded iteration 0 reads no counters, so at log2m 10 it takes 8 lanes
-// per node (4 slabs each) instead of 16: half the lanes wait on each short
-// row (headline iteration 0: 7.17 -> 6.54 ms at r = 6, 7.02 -> 6.07 at r = 5).
+// The seeded iteration 0 reads no counters, so from log2m 10 to 12 it takes
+// half the lanes per node, with twice the slabs each: half as many lanes
+// wait on each short row (log2m 10, r = 6: 7.17 -> 6.54 ms; log2m 12, r = 5:
+// 22.3 -> 16.4 ms; a quarter of the lanes was slower).
+#define HB_SEED_SHAPES(X)                                                                        \
+  X(5, 10, 8, 4, 1) X(6, 10, 8, 4, 1) X(5, 11, 16, 4, 1) X(6, 11, 16, 4, 1) X(5, 12, 32, 4, 1)  \
+  X(6, 12, 32, 4, 1)
+
 hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s) {
-  if ((a.flags & F_SEEDED) && log2m == 10) {
-    if (r == 5) return launch_union_t<5, 8, 4, 1>(a, grid, s);
-    if (r == 6) return launch_union_t<6, 8, 4, 1>(a, grid, s);
-  }
+#define X(RR, LM, GG, SS, NS) \
+  if ((a.flags & F_SEEDED) && r == RR && log2m == LM) return launch_union_t<RR, GG, SS, NS>(a, grid, s);
+  HB_SEED_SHAPES(X)
+#undef X
 #define X(RR, LM, GG, SS, NS) \
   if (r == RR && log2m == LM) return launch_union_t<RR, GG, SS, NS>(a, grid, s);
   HB_SHAPES(X)
