@@ -117,6 +117,7 @@ struct Instance {
   std::vector<uint32_t> heavy_nodes;
   std::vector<uint64_t> heavy_degs;
   DevBuf<uint32_t> d_heavy;
+  DevBuf<uint32_t> d_heavy_sel;  // worklist iterations: the heavy rows with work
   uint64_t heavy_count = 0;
 
   // Dense iterations visit nodes in `order`: ids sorted by decreasing
@@ -431,6 +432,7 @@ struct Instance {
     ua.heavy_degree = heavy_count ? heavy_degree : UINT64_MAX;
     ua.heavy_list = d_heavy.p;
     ua.heavy_count = heavy_count;
+    ua.heavy_len = nullptr;
     ua.chg_cur = chg[mcur].p;
     ua.chg_next = chg[1 - mcur].p;
     ua.chg_shift = chg_shift;
@@ -683,6 +685,14 @@ struct Instance {
     }
     if (p.list_mode && !window_now)
       HB_HIP(launch_compact(p.ua.must, p.ua.curr_mod, n, bsum.p, list.p, list_len.p, stream));
+    if (!window_now && p.ua.heavy_count && !(p.ua.flags & F_CHECK_ALL)) {
+      // worklist iteration: only the heavy rows to check or to copy
+      uint32_t *len = work.p + size_t(kHeavySelCounter) * kWorkStride;
+      HB_HIP(launch_select_heavy(d_heavy.p, p.ua.heavy_count, p.ua.must, p.ua.curr_mod,
+                                 d_heavy_sel.p, len, stream));
+      p.ua.heavy_list = d_heavy_sel.p;
+      p.ua.heavy_len = len;
+    }
     if (timing) HB_HIP(hipEventRecord(ev[1], stream));
     if (window_now)
       HB_HIP(launch_window_union(int(r), int(log2m), p.ua, stream));
@@ -1186,6 +1196,7 @@ std::unique_ptr<Instance> make_instance(const hb_csr *graph, const hb_csr *trans
         I.heavy_nodes.push_back(h.second);
       }
       upload(I.d_heavy, I.heavy_nodes.data(), I.heavy_nodes.size(), s);
+      I.d_heavy_sel.alloc(I.heavy_nodes.size());
       I.set_heavy_degree(Instance::kDefaultHeavy);
     }
     I.stats.alloc(1);

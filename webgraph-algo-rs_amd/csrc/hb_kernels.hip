@@ -1026,16 +1026,18 @@ struct UnionLds {
 
 // Work counter [8] of a.work hands out the heavy rows.  Every workgroup takes
 // rows until none is left (the heaviest first: the list is sorted by
-// decreasing degree), then turns to the chunk queues.  Returns when no heavy
-// row is left; uniform over the workgroup.
+// decreasing degree; in worklist iterations the list holds only the selected
+// rows, in no particular order), then turns to the chunk queues.  Returns
+// when no heavy row is left; uniform over the workgroup.
 template <int R, int G, int SPL>
 __device__ __forceinline__ void heavy_phase(const UnionArgs &a, UnionLds<R, G, SPL> &L, Acc &acc) {
+  const uint64_t count = a.heavy_len ? uint64_t(*a.heavy_len) : a.heavy_count;
   for (;;) {
     if (threadIdx.x == 0) L.u.heavy.h = atomicAdd(a.work + 8 * kWorkStride, 1u);
     __syncthreads();
     const uint64_t h = L.u.heavy.h;
     __syncthreads();
-    if (h >= a.heavy_count) break;
+    if (h >= count) break;
     heavy_node<R, G, SPL>(a, a.heavy_list[h], L.u.heavy.acc, L.ids, L.u.heavy.sum, acc);
   }
 }
@@ -1704,6 +1706,39 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total) {
   __syncthreads();
   *total = tot;
   return before + inc - v;
+}
+
+// Worklist iterations skip most heavy rows: this keeps those whose node must
+// be checked or was modified, so that union_kernel's heavy phase does not
+// line its workgroups up on the row counter while they skip the rest.  One
+// atomic per wavefront appends its selected rows.
+__global__ __launch_bounds__(kBlock) void select_heavy_kernel(const uint32_t *heavy_list,
+                                                              uint64_t heavy_count,
+                                                              const uint32_t *must,
+                                                              const uint32_t *curr_mod,
+                                                              uint32_t *out, uint32_t *out_len) {
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  for (uint64_t base = uint64_t(blockIdx.x) * kBlock + (threadIdx.x & ~63u); base < heavy_count;
+       base += stride) {
+    const uint64_t h = base + __lane_id();
+    uint32_t v = 0;
+    bool sel = false;
+    if (h < heavy_count) {
+      v = heavy_list[h];
+      sel = test_bit(must, v) || test_bit(curr_mod, v);
+    }
+    uint32_t x = sel ? 1u : 0u;  // inclusive prefix count over the wavefront
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (__lane_id() >= uint32_t(o)) x += y;
+    }
+    const uint32_t total = __shfl(x, 63);
+    uint32_t at = 0;
+    if (__lane_id() == 0 && total) at = atomicAdd(out_len, total);
+    at = __shfl(at, 0);
+    if (sel) out[at + x - 1] = v;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void compact_count_kernel(const uint32_t *a, const uint32_t *b,
@@ -2576,6 +2611,15 @@ hipError_t launch_reduce_stats(const Stats *stats, const uint64_t *slots, Stats 
 uint64_t compact_scratch_words(uint64_t n) {
   const uint64_t nw = (n + 31) / 32;
   return (nw + kBlock - 1) / kBlock + 1;
+}
+
+hipError_t launch_select_heavy(const uint32_t *heavy_list, uint64_t heavy_count,
+                               const uint32_t *must, const uint32_t *curr_mod, uint32_t *out,
+                               uint32_t *out_len, hipStream_t s) {
+  if (!heavy_count) return hipSuccess;
+  select_heavy_kernel<<<grid_for(heavy_count, kBlock, 1024), kBlock, 0, s>>>(
+      heavy_list, heavy_count, must, curr_mod, out, out_len);
+  return hipGetLastError();
 }
 
 hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *bsum,

@@ -70,9 +70,10 @@ constexpr int kStatSlotWords = 16;  // u64 per slot
 constexpr int kStatFields = 11;
 
 // Dynamic work counters, each on its own 256-byte line: [0, 8) one per XCD,
-// [8] heavy rows.
+// [8] heavy rows, [9] heavy rows selected for a worklist iteration.
 constexpr int kWorkStride = 64;
-constexpr int kWorkCounters = 9;
+constexpr int kWorkCounters = 10;
+constexpr int kHeavySelCounter = 9;  // select_heavy_kernel's count of selected heavy rows
 
 inline double stats_nf(const Stats &s) {
   const int64_t hi = s.nf_hi + (s.nf_lo >> 32);
@@ -121,6 +122,9 @@ struct UnionArgs {
   uint64_t heavy_degree;         // rows longer than this take a workgroup each (heavy_phase)
   const uint32_t *heavy_list;    // the nodes with such rows
   uint64_t heavy_count;
+  // nullable: the number of rows in heavy_list lives here on the device
+  // (worklist iterations: select_heavy_kernel keeps the rows with work)
+  const uint32_t *heavy_len;
   // Change masks: bit (slab >> chg_shift) of chg_cur[s] is set when a
   // register of that slab of node s changed in the last iteration (or, in
   // iteration 0, is not zero).  Read for modified successors, written for
@@ -265,6 +269,12 @@ hipError_t launch_prepare_iteration(uint32_t *next_mod, uint32_t *next_must, uin
 hipError_t launch_reduce_stats(const Stats *stats, const uint64_t *slots, Stats *out,
                                hipStream_t s);
 // list <- ids of the set bits of (a | b) below n, ascending; *len <- count.
+// Worklist iterations: the heavy rows to check or to copy (must or modified
+// bit set), appended to out in any order, their number at *out_len (zeroed
+// before the launch).
+hipError_t launch_select_heavy(const uint32_t *heavy_list, uint64_t heavy_count,
+                               const uint32_t *must, const uint32_t *curr_mod, uint32_t *out,
+                               uint32_t *out_len, hipStream_t s);
 hipError_t launch_compact(const uint32_t *a, const uint32_t *b, uint64_t n, uint32_t *block_sums,
                           uint32_t *list, uint32_t *len, hipStream_t s);
 uint64_t compact_scratch_words(uint64_t n);
