@@ -2398,13 +2398,14 @@ bool union_shape(int r, int log2m, int *lanes, int *spl, int *nseg) {
   return false;
 }
 
-// The seeded iteration 0 reads no counters, so from log2m 10 to 12 it takes
-// half the lanes per node, with twice the slabs each: half as many lanes
-// wait on each short row (log2m 10, r = 6: 7.17 -> 6.54 ms; log2m 12, r = 5:
-// 22.3 -> 16.4 ms; a quarter of the lanes was slower).
+// The seeded iteration 0 reads no counters, so at log2m 8 and from log2m 10
+// to 12 it takes half the lanes per node, with twice the slabs each: half as
+// many lanes wait on each short row (log2m 8, r = 6: 3.84 -> 3.32 ms; log2m
+// 10, r = 6: 7.17 -> 6.54 ms; log2m 12, r = 5: 22.3 -> 16.4 ms; a quarter of
+// the lanes was slower at log2m 10, one lane instead of two at log2m 6 too).
 #define HB_SEED_SHAPES(X)                                                                        \
-  X(5, 10, 8, 4, 1) X(6, 10, 8, 4, 1) X(5, 11, 16, 4, 1) X(6, 11, 16, 4, 1) X(5, 12, 32, 4, 1)  \
-  X(6, 12, 32, 4, 1)
+  X(5, 8, 4, 2, 1) X(6, 8, 4, 2, 1) X(5, 10, 8, 4, 1) X(6, 10, 8, 4, 1) X(5, 11, 16, 4, 1)      \
+  X(6, 11, 16, 4, 1) X(5, 12, 32, 4, 1) X(6, 12, 32, 4, 1)
 
 hipError_t launch_union(int r, int log2m, const UnionArgs &a, uint32_t grid, hipStream_t s) {
 #define X(RR, LM, GG, SS, NS) \
